@@ -1,0 +1,265 @@
+"""MI355X-native ForceAtlas2 engine: Python access to libgfa2.so through its C ABI.
+
+The engine itself is HIP (``csrc/*.hip``) behind ``include/gfa2.h``; this module only
+binds that ABI with ctypes for tests, benchmarks and scripts. There is no fallback: if
+the shared library is missing or no HIP device is present, calls raise.
+
+Reference interface mirrored (govertb/GPUGraphLayout): ``RPGraph::ForceAtlas2``
+(src/RPForceAtlas2.hpp:32-60) and ``CUDAForceAtlas2`` (src/RPGPUForceAtlas2.hpp:30-68):
+``do_step``/``do_steps``, ``sync_layout`` (positions out), ``set_scale``/``set_gravity``
+via parameters.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgfa2.so")
+
+GFA2_FORCE_TOTAL = 0
+GFA2_FORCE_REPULSION = 1
+GFA2_FORCE_ATTRACTION_GRAVITY = 2
+
+# Every symbol include/gfa2.h declares.
+ABI_SYMBOLS = (
+    "gfa2_default_params", "gfa2_create", "gfa2_set_graph", "gfa2_set_graph_coo",
+    "gfa2_set_positions", "gfa2_get_positions", "gfa2_reset", "gfa2_step", "gfa2_sync",
+    "gfa2_get_forces", "gfa2_set_capture", "gfa2_get_tree", "gfa2_get_stats",
+    "gfa2_set_timing", "gfa2_get_timing", "gfa2_set_graph_capture", "gfa2_destroy",
+    "gfa2_strerror", "gfa2_device_info",
+)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("use_barneshut", ctypes.c_int32),
+        ("strong_gravity", ctypes.c_int32),
+        ("gravity", ctypes.c_float),
+        ("scale", ctypes.c_float),
+        ("theta", ctypes.c_float),
+        ("epssq", ctypes.c_float),
+        ("jitter_tolerance", ctypes.c_float),
+        ("k_s_max", ctypes.c_float),
+        ("global_speed", ctypes.c_float),
+        ("speed_efficiency", ctypes.c_float),
+        ("group_size", ctypes.c_int32),
+        ("cell_factor", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("iteration", ctypes.c_int32),
+        ("nodes", ctypes.c_int32),
+        ("edges", ctypes.c_int64),
+        ("global_speed", ctypes.c_float),
+        ("speed_efficiency", ctypes.c_float),
+        ("swing", ctypes.c_double),
+        ("traction", ctypes.c_double),
+        ("interactions", ctypes.c_int64),
+        ("cells", ctypes.c_int32),
+        ("max_level", ctypes.c_int32),
+        ("root_x", ctypes.c_float),
+        ("root_y", ctypes.c_float),
+        ("radius", ctypes.c_float),
+        ("error_flags", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GFA2Error(RuntimeError):
+    def __init__(self, code: int, what: str):
+        self.code = code
+        super().__init__(f"{what}: {lib().gfa2_strerror(code).decode()} ({code})")
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libgfa2.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    L.gfa2_default_params.argtypes = [ctypes.POINTER(Params)]
+    L.gfa2_default_params.restype = None
+    L.gfa2_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(_P)]
+    L.gfa2_set_graph.argtypes = [_P, _I32, _P, _P, _P]
+    L.gfa2_set_graph_coo.argtypes = [_P, _I32, _I64, _P, _P, _P]
+    L.gfa2_set_positions.argtypes = [_P, _P]
+    L.gfa2_get_positions.argtypes = [_P, _P]
+    L.gfa2_reset.argtypes = [_P]
+    L.gfa2_step.argtypes = [_P, _I32]
+    L.gfa2_sync.argtypes = [_P]
+    L.gfa2_get_forces.argtypes = [_P, _I32, _P]
+    L.gfa2_set_capture.argtypes = [_P, _I32]
+    L.gfa2_get_tree.argtypes = [_P, _I32, _P, _P, _P, _P, _P, _P]
+    L.gfa2_get_stats.argtypes = [_P, ctypes.POINTER(Stats)]
+    L.gfa2_set_timing.argtypes = [_P, _I32]
+    L.gfa2_get_timing.argtypes = [_P, _I32, _P, ctypes.POINTER(ctypes.c_char_p), _P]
+    L.gfa2_set_graph_capture.argtypes = [_P, _I32]
+    L.gfa2_destroy.argtypes = [_P]
+    L.gfa2_strerror.argtypes = [ctypes.c_int]
+    L.gfa2_strerror.restype = ctypes.c_char_p
+    L.gfa2_device_info.argtypes = [ctypes.c_int, _P, _P, _P]
+    for name in ABI_SYMBOLS:
+        fn = getattr(L, name)
+        if name not in ("gfa2_default_params", "gfa2_strerror"):
+            fn.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise GFA2Error(rc, what)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().gfa2_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def coo_to_csr(n: int, src: np.ndarray, dst: np.ndarray):
+    """Symmetric CSR (int64 rowptr, int32 col) from an undirected COO list, each edge once.
+    Row order follows the COO order, as gfa2_set_graph_coo builds it."""
+    src = np.asarray(src, dtype=np.int64)
+    dst = np.asarray(dst, dtype=np.int64)
+    rows = np.concatenate([np.stack([src, dst], 1), np.stack([dst, src], 1)], 0)
+    # within a row, entries keep the COO order of their edge
+    key = np.concatenate([np.arange(len(src)) * 2, np.arange(len(src)) * 2 + 1])
+    order = np.lexsort((key, rows[:, 0]))
+    r = rows[order]
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(rowptr, r[:, 0] + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int64)
+    return rowptr, r[:, 1].astype(np.int32)
+
+
+class Engine:
+    """One layout instance on one GPU (gfa2_handle)."""
+
+    def __init__(self, n: int, src=None, dst=None, rowptr=None, col=None, mass=None, device: int = 0, **params):
+        self.params = default_params(**params)
+        h = ctypes.c_void_p()
+        _check(lib().gfa2_create(ctypes.byref(self.params), device, ctypes.byref(h)), "gfa2_create")
+        self.h = h
+        self.n = int(n)
+        m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float32)
+        if rowptr is not None:
+            self._rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+            self._col = np.ascontiguousarray(col, dtype=np.int32)
+            _check(lib().gfa2_set_graph(self.h, self.n, _ptr(self._rowptr), _ptr(self._col), _ptr(m)),
+                   "gfa2_set_graph")
+        else:
+            s = np.ascontiguousarray(src, dtype=np.int32)
+            d = np.ascontiguousarray(dst, dtype=np.int32)
+            _check(lib().gfa2_set_graph_coo(self.h, self.n, len(s), _ptr(s), _ptr(d), _ptr(m)),
+                   "gfa2_set_graph_coo")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gfa2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_positions(self, xy: np.ndarray):
+        a = np.ascontiguousarray(xy, dtype=np.float32).reshape(self.n, 2)
+        _check(lib().gfa2_set_positions(self.h, _ptr(a)), "gfa2_set_positions")
+
+    def positions(self) -> np.ndarray:
+        out = np.empty((self.n, 2), dtype=np.float32)
+        _check(lib().gfa2_get_positions(self.h, _ptr(out)), "gfa2_get_positions")
+        return out
+
+    sync_layout = positions
+
+    def reset(self):
+        _check(lib().gfa2_reset(self.h), "gfa2_reset")
+
+    def step(self, iters: int = 1):
+        _check(lib().gfa2_step(self.h, int(iters)), "gfa2_step")
+
+    do_steps = step
+
+    def do_step(self):
+        self.step(1)
+
+    def sync(self):
+        _check(lib().gfa2_sync(self.h), "gfa2_sync")
+
+    def forces(self, kind: int = GFA2_FORCE_TOTAL) -> np.ndarray:
+        out = np.empty((self.n, 2), dtype=np.float32)
+        _check(lib().gfa2_get_forces(self.h, int(kind), _ptr(out)), "gfa2_get_forces")
+        return out
+
+    def set_capture(self, on: bool = True):
+        _check(lib().gfa2_set_capture(self.h, int(bool(on))), "gfa2_set_capture")
+
+    def set_graph_capture(self, on: bool = True):
+        _check(lib().gfa2_set_graph_capture(self.h, int(bool(on))), "gfa2_set_graph_capture")
+
+    def tree(self):
+        """Cells in pre-order: dict of level, count, mass, xy; plus the in-order permutation."""
+        nc = ctypes.c_int32(0)
+        _check(lib().gfa2_get_tree(self.h, 0, ctypes.byref(nc), None, None, None, None, None), "gfa2_get_tree")
+        c = nc.value
+        level = np.empty(c, np.int32)
+        count = np.empty(c, np.int32)
+        mass = np.empty(c, np.float32)
+        xy = np.empty((c, 2), np.float32)
+        perm = np.empty(self.n, np.int32)
+        _check(lib().gfa2_get_tree(self.h, c, ctypes.byref(nc), _ptr(level), _ptr(count), _ptr(mass), _ptr(xy),
+                                   _ptr(perm)), "gfa2_get_tree")
+        return {"level": level, "count": count, "mass": mass, "xy": xy, "perm": perm}
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().gfa2_get_stats(self.h, ctypes.byref(s)), "gfa2_get_stats")
+        return s.as_dict()
+
+    def set_timing(self, on: bool = True):
+        _check(lib().gfa2_set_timing(self.h, int(bool(on))), "gfa2_set_timing")
+
+    def timing(self) -> dict:
+        names = (ctypes.c_char_p * 32)()
+        ms = np.zeros(32, dtype=np.float64)
+        cnt = ctypes.c_int32(0)
+        _check(lib().gfa2_get_timing(self.h, 32, ctypes.byref(cnt), names, _ptr(ms)), "gfa2_get_timing")
+        return {names[i].decode(): float(ms[i]) for i in range(cnt.value)}
+
+
+def device_info(device: int = 0) -> dict:
+    cus = ctypes.c_int32(0)
+    wave = ctypes.c_int32(0)
+    mem = ctypes.c_int64(0)
+    _check(lib().gfa2_device_info(device, ctypes.byref(cus), ctypes.byref(wave), ctypes.byref(mem)),
+           "gfa2_device_info")
+    return {"cus": cus.value, "wave": wave.value, "mem_bytes": mem.value}

@@ -1,0 +1,422 @@
+// Barnes-Hut quadtree construction for gfx950, deterministic and lock-free.
+//
+// The reference builds its quadtree by concurrent insertion with CAS locks and then
+// summarises and sorts it with spin-waiting kernels (src/RPBHKernels.cu:184-559). Here
+// the same tree is derived from sorted path keys instead:
+//  1. bounding box (the reference's BoundingBoxKernel, src/RPBHKernels.cu:70-157), from
+//     per-block partials every key block reduces redundantly;
+//  2. per body a 64-bit key of 32 quadrant digits (bit0: x, bit1: y), obtained by
+//     replaying the reference's float centre arithmetic along the body's path
+//     (r *= 0.5f, x += +/-r; src/RPBHKernels.cu:217-241, :292-308). A quadrant tree is
+//     fixed by the bodies' paths, whatever the insertion order;
+//  3. radix sort of the keys: the sorted order is the reference's in-order body
+//     permutation (SortKernel, src/RPBHKernels.cu:513-559);
+//  4. with lcp_i the common digits of sorted neighbours i and i+1, the cells are exactly
+//     the prefixes of length <= lcp_i; the cells starting at position i have levels
+//     lcp_{i-1}+1 .. lcp_i. Their pre-order record index follows from an exclusive scan
+//     of those counts, and each cell's extent from a galloping search over the keys;
+//  5. summaries bottom-up, one launch per level, children in quadrant order, mass
+//     accumulated from 0.0f and centroid (sum m*p)*(1/mass), as the reference's
+//     SummarizationKernel (src/RPBHKernels.cu:357-506). Counts are extents.
+// Nothing waits on another workgroup; every dependency crosses a launch boundary.
+#include "engine_internal.h"
+
+namespace gfa2 {
+namespace {
+
+__device__ __forceinline__ int lcp_levels(unsigned long long a, unsigned long long b) {
+    unsigned long long x = a ^ b;
+    return x == 0ull ? kKeyLevels : (__clzll((long long)x) >> 1);
+}
+
+// Reduce bounding-box partials {minx, miny, maxx, maxy} in LDS (every block does it).
+__device__ void reduce_bbox(const float4* __restrict__ part, int np, float* root /*[3] LDS*/) {
+    __shared__ float4 red[256];
+    const int tid = threadIdx.x;
+    float4 acc = part[0];
+    for (int k = tid; k < np; k += blockDim.x) {
+        float4 v = part[k];
+        acc.x = fminf(acc.x, v.x);
+        acc.y = fminf(acc.y, v.y);
+        acc.z = fmaxf(acc.z, v.z);
+        acc.w = fmaxf(acc.w, v.w);
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (tid < o) {
+            float4 v = red[tid + o];
+            acc.x = fminf(acc.x, v.x);
+            acc.y = fminf(acc.y, v.y);
+            acc.z = fmaxf(acc.z, v.z);
+            acc.w = fmaxf(acc.w, v.w);
+            red[tid] = acc;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        float4 t = red[0];
+        root[2] = fmaxf(t.z - t.x, t.w - t.y) * 0.5f;   // src/RPBHKernels.cu:140
+        root[0] = (t.x + t.z) * 0.5f;                    // :147
+        root[1] = (t.y + t.w) * 0.5f;                    // :148
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_bbox_partials(const float2* __restrict__ pos, int n,
+                                                       float4* __restrict__ part, DevState* st) {
+    __shared__ float4 red[256];
+    const int tid = threadIdx.x;
+    float2 p0 = pos[0];
+    float4 acc = make_float4(p0.x, p0.y, p0.x, p0.y);
+    for (int i = blockIdx.x * 256 + tid; i < n; i += gridDim.x * 256) {
+        float2 p = pos[i];
+        acc.x = fminf(acc.x, p.x);
+        acc.y = fminf(acc.y, p.y);
+        acc.z = fmaxf(acc.z, p.x);
+        acc.w = fmaxf(acc.w, p.y);
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            float4 v = red[tid + o];
+            acc.x = fminf(acc.x, v.x);
+            acc.y = fminf(acc.y, v.y);
+            acc.z = fmaxf(acc.z, v.z);
+            acc.w = fmaxf(acc.w, v.w);
+            red[tid] = acc;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        part[blockIdx.x] = red[0];
+        if (blockIdx.x == 0) st->nbbox = gridDim.x;
+    }
+}
+
+// Root box + path keys. vals = identity.
+__global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, int n,
+                                              const float4* __restrict__ part, DevState* st,
+                                              unsigned long long* __restrict__ keys,
+                                              unsigned int* __restrict__ vals) {
+    __shared__ float root[3];
+    reduce_bbox(part, st->nbbox, root);
+    const float rootx = root[0], rooty = root[1], radius = root[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rootx = rootx;
+        st->rooty = rooty;
+        st->radius = radius;
+    }
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const float2 p = pos[i];
+        float r = radius * 0.5f;
+        bool bx = rootx < p.x, by = rooty < p.y;
+        float x = rootx + (bx ? r : -r);
+        float y = rooty + (by ? r : -r);
+        unsigned long long key = (unsigned long long)((unsigned)bx | ((unsigned)by << 1)) << 62;
+#pragma unroll
+        for (int l = 1; l < kKeyLevels; ++l) {
+            r *= 0.5f;
+            bx = x < p.x;
+            by = y < p.y;
+            key |= (unsigned long long)((unsigned)bx | ((unsigned)by << 1)) << (62 - 2 * l);
+            x += bx ? r : -r;
+            y += by ? r : -r;
+        }
+        keys[i] = key;
+        vals[i] = (unsigned)i;
+    }
+}
+
+__device__ __forceinline__ int lcp_at(const unsigned long long* __restrict__ keys, int n, int i) {
+    if (i < 0) return -1;
+    if (i >= n - 1) return n == 1 ? 0 : -1;
+    return lcp_levels(keys[i], keys[i + 1]);
+}
+
+// Block-wide exclusive scan of one int per thread (256 threads). Returns the exclusive
+// prefix; *total gets the block sum.
+__device__ int block_exclusive_scan(int v, int* total) {
+    __shared__ int wsum[kTreeThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kTreeThreads / 64; ++w) {
+        int s = wsum[w];
+        if (w < wave) before += s;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+// lcp per sorted position and the number of cells opened there; block sums.
+__global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* __restrict__ keys, int n,
+                                                      signed char* __restrict__ lcp,
+                                                      int* __restrict__ blocksum) {
+    const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
+    int prev = lcp_at(keys, n, i0 - 1);
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kTreeItems; ++k) {
+        int i = i0 + k;
+        if (i < n) {
+            int l = lcp_at(keys, n, i);
+            lcp[i] = (signed char)l;
+            sum += max(0, l - prev);
+            prev = l;
+        }
+    }
+    int total;
+    block_exclusive_scan(sum, &total);
+    if (threadIdx.x == 0) blocksum[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of block sums (in place), cell total, capacity check,
+// per-iteration counters reset.
+__global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum, int nb, int n,
+                                                      int cap_records, DevState* st) {
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = tid * per;
+    int local = 0;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nb) local += blocksum[b0 + k];
+    int inc = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < 16; ++w) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    int run = before + inc - local;
+    for (int k = 0; k < per; ++k) {
+        if (b0 + k < nb) {
+            int c = blocksum[b0 + k];
+            blocksum[b0 + k] = run;
+            run += c;
+        }
+    }
+    if (tid == 0) {
+        st->ncells = all;
+        st->nrecords = n + all;
+        if (n + all > cap_records) st->err |= kErrTreeOverflow;
+        st->max_level = 0;
+    }
+    if (tid <= kMaxLevels) {
+        st->level_count[tid] = 0;
+        st->level_cursor[tid] = 0;
+    }
+}
+
+// S (exclusive prefix of opened cells), body records and brec.
+__global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __restrict__ lcp, int n,
+                                                         const int* __restrict__ blockoff,
+                                                         int* __restrict__ S,
+                                                         const unsigned int* __restrict__ vals,
+                                                         const float2* __restrict__ pos,
+                                                         const float* __restrict__ mass,
+                                                         float4* __restrict__ rec, int* __restrict__ brec,
+                                                         const DevState* __restrict__ st) {
+    const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
+    int s[kTreeItems];
+    int prev = (i0 - 1 >= 0 && i0 - 1 < n) ? (int)lcp[i0 - 1] : -1;
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kTreeItems; ++k) {
+        int i = i0 + k;
+        int l = i < n ? (int)lcp[i] : prev;
+        s[k] = max(0, l - prev);
+        prev = l;
+        sum += s[k];
+    }
+    int total;
+    int run = block_exclusive_scan(sum, &total) + blockoff[blockIdx.x];
+    const bool ok = st->err == 0;
+#pragma unroll
+    for (int k = 0; k < kTreeItems; ++k) {
+        int i = i0 + k;
+        if (i < n) {
+            S[i] = run;
+            run += s[k];
+            int rb = i + run;          // after this position's own cells
+            brec[i] = rb;
+            if (ok) {
+                unsigned int id = vals[i];
+                float2 p = pos[id];
+                rec[rb] = make_float4(p.x, p.y, mass[id], __int_as_float(-1));
+            }
+        }
+    }
+    if (i0 + kTreeItems >= n && i0 < n) S[n] = run;   // total cells
+}
+
+// Cell records (skip, level, count) and the cell list for level bucketing.
+__global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long* __restrict__ keys,
+                                                        const signed char* __restrict__ lcp, int n,
+                                                        const int* __restrict__ S, float4* __restrict__ rec,
+                                                        int* __restrict__ rcount, int* __restrict__ cell_rec,
+                                                        unsigned char* __restrict__ cell_lvl, DevState* st) {
+    __shared__ int hist[kMaxLevels + 1];
+    __shared__ int lmax;
+    if (threadIdx.x <= kMaxLevels) hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) lmax = 0;
+    __syncthreads();
+    const bool ok = st->err == 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ok && i < n) {
+        const int lprev = i > 0 ? (int)lcp[i - 1] : -1;
+        const int lcur = (int)lcp[i];
+        if (lcur > lprev) {
+            const unsigned long long ki = keys[i];
+            const int Si = S[i];
+            int lo = i;     // known member of the current cell's extent
+            for (int L = lcur; L > lprev; --L) {
+                // extent end: largest j with lcp(k_i, k_j) >= L (keys sorted => contiguous)
+                int step = 1;
+                int hi = lo + 1;
+                while (hi < n && lcp_levels(ki, keys[hi]) >= L) {
+                    lo = hi;
+                    step <<= 1;
+                    hi = lo + step;
+                }
+                if (hi > n) hi = n;
+                while (hi - lo > 1) {
+                    int mid = lo + ((hi - lo) >> 1);
+                    if (lcp_levels(ki, keys[mid]) >= L) lo = mid;
+                    else hi = mid;
+                }
+                const int b = lo;
+                const int ord = Si + (L - lprev - 1);
+                const int rc = i + ord;
+                const unsigned int skip = (unsigned int)(b + S[b + 1] + 1 - rc);
+                const unsigned int info = (skip << kInfoLevelBits) | (unsigned int)((L > 0 ? L - 1 : 0) & 31);
+                rec[rc].w = __uint_as_float(info);
+                rcount[rc] = b - i + 1;
+                cell_rec[ord] = rc;
+                cell_lvl[ord] = (unsigned char)L;
+                atomicAdd(&hist[L], 1);
+            }
+            atomicMax(&lmax, lcur);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x <= kMaxLevels && hist[threadIdx.x] != 0) atomicAdd(&st->level_count[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x == 0 && lmax > 0) atomicMax(&st->max_level, lmax);
+}
+
+// Bucket cells by level (order inside a level is irrelevant).
+__global__ __launch_bounds__(256) void k_levels(const int* __restrict__ cell_rec,
+                                                const unsigned char* __restrict__ cell_lvl,
+                                                int* __restrict__ level_list, DevState* st) {
+    __shared__ int off[kMaxLevels + 2];
+    __shared__ int hist[kMaxLevels + 1];
+    __shared__ int base[kMaxLevels + 1];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        int run = 0;
+        for (int L = 0; L <= kMaxLevels; ++L) {
+            off[L] = run;
+            run += st->level_count[L];
+        }
+        off[kMaxLevels + 1] = run;
+        if (blockIdx.x == 0)
+            for (int L = 0; L <= kMaxLevels + 1; ++L) st->level_offset[L] = off[L];
+    }
+    if (tid <= kMaxLevels) hist[tid] = 0;
+    __syncthreads();
+    const int C = st->err == 0 ? st->ncells : 0;
+    const int t = blockIdx.x * 256 + tid;
+    int L = -1, local = 0, rc = 0;
+    if (t < C) {
+        L = cell_lvl[t];
+        rc = cell_rec[t];
+        local = atomicAdd(&hist[L], 1);
+    }
+    __syncthreads();
+    if (tid <= kMaxLevels && hist[tid] != 0) base[tid] = atomicAdd(&st->level_cursor[tid], hist[tid]);
+    __syncthreads();
+    if (L >= 0) level_list[off[L] + base[L] + local] = rc;
+}
+
+// Summaries of all cells at one level; children are the records between rc+1 and
+// rc+skip, hopped by their own skips, in quadrant order.
+__global__ __launch_bounds__(256) void k_summarize(int L, const int* __restrict__ level_list,
+                                                   float4* __restrict__ rec, const DevState* __restrict__ st) {
+    if (st->err != 0) return;
+    const int beg = st->level_offset[L];
+    const int cnt = st->level_count[L];
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < cnt; t += gridDim.x * 256) {
+        const int rc = level_list[beg + t];
+        const unsigned int info = __float_as_uint(rec[rc].w);
+        const int end = min(rc + (int)(info >> kInfoLevelBits), st->nrecords);
+        float cm = 0.0f, px = 0.0f, py = 0.0f;
+        int r = rc + 1;
+        while (r < end) {
+            const float4 c = rec[r];
+            const unsigned int ci = __float_as_uint(c.w);
+            px = fmaf(c.x, c.z, px);
+            py = fmaf(c.y, c.z, py);
+            cm += c.z;
+            r += ci == 0xffffffffu ? 1 : max((int)(ci >> kInfoLevelBits), 1);
+        }
+        const float inv = 1.0f / cm;
+        rec[rc].x = px * inv;
+        rec[rc].y = py * inv;
+        rec[rc].z = cm;
+    }
+}
+
+}  // namespace
+
+int displacement_blocks(int n) {
+    int b = (n + 255) / 256;
+    return b < 1 ? 1 : (b > kPartBlocks ? kPartBlocks : b);
+}
+
+void launch_bbox_partials(const Bufs& b, hipStream_t s) {
+    k_bbox_partials<<<displacement_blocks(b.n), 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st);
+}
+
+void launch_keys(const Bufs& b, hipStream_t s) {
+    int blocks = (b.n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    k_keys<<<blocks, 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st, b.sort.keys[0], b.sort.vals[0]);
+}
+
+void launch_tree(Bufs& b, hipStream_t s) {
+    const int n = b.n;
+    const int tiles = (n + kTreeTile - 1) / kTreeTile;
+    k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], n, b.lcp, b.blocksum);
+    k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.st);
+    k_prefix<<<tiles, kTreeThreads, 0, s>>>(b.lcp, n, b.blocksum, b.S, b.sort.vals[0], b.pos, b.mass, b.rec,
+                                            b.brec, b.st);
+    k_nodes<<<(n + kTreeThreads - 1) / kTreeThreads, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, n, b.S, b.rec,
+                                                                           b.rcount, b.cell_rec, b.cell_lvl, b.st);
+    const int max_cells = b.cap_records - n;
+    k_levels<<<(max_cells + 255) / 256, 256, 0, s>>>(b.cell_rec, b.cell_lvl, b.level_list, b.st);
+    int sum_blocks = (max_cells / 4 + 255) / 256;
+    if (sum_blocks < 1) sum_blocks = 1;
+    if (sum_blocks > 1024) sum_blocks = 1024;
+    for (int L = kKeyLevels; L >= 0; --L)
+        k_summarize<<<sum_blocks, 256, 0, s>>>(L, b.level_list, b.rec, b.st);
+}
+
+}  // namespace gfa2

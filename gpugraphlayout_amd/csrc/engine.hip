@@ -1,0 +1,545 @@
+// The engine behind include/gfa2.h: one handle = one layout instance with its own HIP
+// stream, device buffers and device-resident FA2 state (the reference keeps that state
+// in __device__ globals, src/RPFA2Kernels.cu:28-33 and src/RPBHKernels.cu:53-62, which
+// allows one instance per process).
+#include "engine_internal.h"
+#include "../../include/gfa2.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace gfa2;
+
+namespace {
+
+const char* const kGroupNames[] = {"attraction", "keys", "sort", "tree", "walk", "speed", "displacement"};
+constexpr int kGroups = 7;
+
+}  // namespace
+
+struct gfa2_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    gfa2_params user{};
+    Params p{};
+    Bufs b{};
+    bool has_graph = false;
+    bool has_pos = false;
+    long long edges = 0;
+    bool capture_rep = false;
+    std::vector<void*> allocs;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> events;     // pool, kGroups+1 per pending iteration
+    int pending = 0;                    // iterations with recorded, unread events
+    double acc_ms[kGroups] = {0};
+    long long timed_iters = 0;
+    // graph capture
+    bool use_graph = false;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return GFA2_OK;
+    if (e == hipErrorOutOfMemory) return GFA2_ERR_OOM;
+    return GFA2_ERR_HIP;
+}
+
+#define GFA2_TRY(expr)                       \
+    do {                                     \
+        hipError_t _e = (expr);              \
+        if (_e != hipSuccess) return hip_status(_e); \
+    } while (0)
+
+template <typename T>
+int dalloc(gfa2_engine* h, T** ptr, size_t count) {
+    void* p = nullptr;
+    size_t bytes = sizeof(T) * (count ? count : 1);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return hip_status(e);
+    h->allocs.push_back(p);
+    *ptr = static_cast<T*>(p);
+    return GFA2_OK;
+}
+
+void free_all(gfa2_engine* h) {
+    if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+    h->graph_valid = false;
+    for (void* p : h->allocs) hipFree(p);
+    h->allocs.clear();
+    h->b = Bufs{};
+    h->has_graph = false;
+    h->has_pos = false;
+}
+
+void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
+    hipStream_t s = h->stream;
+    Bufs& b = h->b;
+    if (ev) hipEventRecord(ev[0], s);
+    launch_attraction(b, h->p, s);
+    if (ev) hipEventRecord(ev[1], s);
+    launch_keys(b, s);
+    if (ev) hipEventRecord(ev[2], s);
+    radix_sort_pairs(b.sort, b.n, 0, 64, s);
+    if (ev) hipEventRecord(ev[3], s);
+    launch_tree(b, s);
+    if (ev) hipEventRecord(ev[4], s);
+    launch_walk(b, h->p, s);
+    if (ev) hipEventRecord(ev[5], s);
+    launch_speed(b, s);
+    if (ev) hipEventRecord(ev[6], s);
+    launch_displacement(b, s);
+    if (ev) hipEventRecord(ev[7], s);
+}
+
+// Reads back the events of completed timed iterations.
+int drain_timing(gfa2_engine* h) {
+    if (h->pending == 0) return GFA2_OK;
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    for (int it = 0; it < h->pending; ++it) {
+        hipEvent_t* ev = &h->events[(size_t)it * (kGroups + 1)];
+        for (int g = 0; g < kGroups; ++g) {
+            float ms = 0.0f;
+            GFA2_TRY(hipEventElapsedTime(&ms, ev[g], ev[g + 1]));
+            h->acc_ms[g] += ms;
+        }
+    }
+    h->timed_iters += h->pending;
+    h->pending = 0;
+    return GFA2_OK;
+}
+
+int reset_state(gfa2_engine* h) {
+    DevState st{};
+    st.global_speed = h->user.global_speed;
+    st.speed_efficiency = h->user.speed_efficiency;
+    st.jitter_tolerance = h->user.jitter_tolerance;
+    st.k_s_max = h->user.k_s_max;
+    GFA2_TRY(hipMemcpyAsync(h->b.st, &st, sizeof(DevState), hipMemcpyHostToDevice, h->stream));
+    GFA2_TRY(hipMemsetAsync(h->b.fprev, 0, sizeof(float2) * h->b.n, h->stream));
+    GFA2_TRY(hipMemsetAsync(h->b.f, 0, sizeof(float2) * h->b.n, h->stream));
+    GFA2_TRY(hipMemsetAsync(h->b.fga, 0, sizeof(float2) * h->b.n, h->stream));
+    if (h->has_pos) launch_bbox_partials(h->b, h->stream);
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    return GFA2_OK;
+}
+
+// Copies DevState back (waits for the stream).
+int read_state(gfa2_engine* h, DevState* st) {
+    GFA2_TRY(hipMemcpyAsync(st, h->b.st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    return GFA2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void gfa2_default_params(gfa2_params* p) {
+    if (!p) return;
+    p->use_barneshut = 1;
+    p->strong_gravity = 0;
+    p->gravity = 1.0f;
+    p->scale = 1.0f;
+    p->theta = 1.0f;
+    p->epssq = (float)(0.05 * 0.05);
+    p->jitter_tolerance = 1.0f;
+    p->k_s_max = 10.0f;
+    p->global_speed = 1.0f;
+    p->speed_efficiency = 1.0f;
+    p->group_size = 32;
+    p->cell_factor = 0;
+}
+
+int gfa2_device_info(int device, int32_t* cus, int32_t* wave, int64_t* mem_bytes) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GFA2_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return GFA2_ERR_INVALID_ARG;
+    hipDeviceProp_t prop;
+    GFA2_TRY(hipGetDeviceProperties(&prop, device));
+    if (cus) *cus = prop.multiProcessorCount;
+    if (wave) *wave = prop.warpSize;
+    if (mem_bytes) *mem_bytes = (int64_t)prop.totalGlobalMem;
+    return GFA2_OK;
+}
+
+int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out) {
+    if (!out) return GFA2_ERR_INVALID_ARG;
+    *out = nullptr;
+    gfa2_params prm;
+    if (params) prm = *params;
+    else gfa2_default_params(&prm);
+    if (!prm.use_barneshut) return GFA2_ERR_UNSUPPORTED;
+    if (prm.group_size != 32 && prm.group_size != 64) return GFA2_ERR_INVALID_ARG;
+    if (!(prm.theta > 0.0f) || !(prm.epssq >= 0.0f) || prm.cell_factor < 0) return GFA2_ERR_INVALID_ARG;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GFA2_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return GFA2_ERR_INVALID_ARG;
+    hipDeviceProp_t prop;
+    GFA2_TRY(hipGetDeviceProperties(&prop, device));
+    // The kernels are written for 64-lane wavefronts (the reference likewise refuses any
+    // warp size but its own, src/RPGPUForceAtlas2.cu:90-95).
+    if (prop.warpSize != 64) return GFA2_ERR_UNSUPPORTED;
+    GFA2_TRY(hipSetDevice(device));
+    gfa2_engine* h = new (std::nothrow) gfa2_engine();
+    if (!h) return GFA2_ERR_OOM;
+    h->device = device;
+    h->user = prm;
+    h->p.strong_gravity = prm.strong_gravity;
+    h->p.k_g = prm.gravity;
+    h->p.k_r = prm.scale;
+    h->p.itolsq = 1.0f / (prm.theta * prm.theta);
+    h->p.epssq = prm.epssq;
+    h->p.group = prm.group_size;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_status(e);
+    }
+    *out = h;
+    return GFA2_OK;
+}
+
+int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_t* col, const float* mass) {
+    if (!h || !rowptr || n < 1) return n < 1 ? GFA2_ERR_INVALID_GRAPH : GFA2_ERR_INVALID_ARG;
+    GFA2_TRY(hipSetDevice(h->device));
+    if (rowptr[0] != 0) return GFA2_ERR_INVALID_GRAPH;
+    const int64_t nnz = rowptr[n];
+    if (nnz < 0 || nnz >= (int64_t)1 << 31 || (nnz > 0 && !col)) return GFA2_ERR_INVALID_GRAPH;
+    std::vector<int> rp32((size_t)n + 1);
+    std::vector<int> lists;
+    lists.reserve(n);
+    std::vector<int> medium, heavy;
+    std::vector<float> m((size_t)n);
+    for (int v = 0; v < n; ++v) {
+        const int64_t a = rowptr[v], b = rowptr[v + 1];
+        if (b < a || b > nnz) return GFA2_ERR_INVALID_GRAPH;
+        for (int64_t e = a; e < b; ++e)
+            if (col[e] < 0 || col[e] >= n || col[e] == v) return GFA2_ERR_INVALID_GRAPH;
+        rp32[v] = (int)a;
+        const int64_t deg = b - a;
+        if (deg <= 32) lists.push_back(v);
+        else if (deg <= 2048) medium.push_back(v);
+        else heavy.push_back(v);
+        m[v] = mass ? mass[v] : (float)(deg + 1.0);
+    }
+    rp32[n] = (int)nnz;
+    const int nlight = (int)lists.size();
+    lists.insert(lists.end(), medium.begin(), medium.end());
+    lists.insert(lists.end(), heavy.begin(), heavy.end());
+
+    free_all(h);
+    const int factor = h->user.cell_factor > 0 ? h->user.cell_factor : 3;
+    long long cap = (long long)n + (long long)factor * n + 1024;
+    if (cap > kMaxRecords) cap = kMaxRecords;
+    if ((long long)n * 3 / 2 + 2 > cap) return GFA2_ERR_UNSUPPORTED;
+    Bufs& b = h->b;
+    b.n = n;
+    b.nnz = nnz;
+    b.cap_records = (int)cap;
+    const int sort_tiles = (n + kSortTile - 1) / kSortTile;
+    const int tree_tiles = (n + kTreeTile - 1) / kTreeTile;
+    const size_t maxcells = (size_t)(cap - n);
+    int st = GFA2_OK;
+#define GFA2_ALLOC(ptr, cnt) \
+    if ((st = dalloc(h, &(ptr), (cnt))) != GFA2_OK) { free_all(h); return st; }
+    GFA2_ALLOC(b.pos, n);
+    GFA2_ALLOC(b.mass, n);
+    GFA2_ALLOC(b.rowptr, (size_t)n + 1);
+    GFA2_ALLOC(b.col, (size_t)nnz);
+    GFA2_ALLOC(b.node_lists, n);
+    GFA2_ALLOC(b.fga, n);
+    GFA2_ALLOC(b.f, n);
+    GFA2_ALLOC(b.fprev, n);
+    GFA2_ALLOC(b.sort.keys[0], n);
+    GFA2_ALLOC(b.sort.keys[1], n);
+    GFA2_ALLOC(b.sort.vals[0], n);
+    GFA2_ALLOC(b.sort.vals[1], n);
+    GFA2_ALLOC(b.sort.hist, (size_t)256 * sort_tiles);
+    GFA2_ALLOC(b.sort.rowtot, 256);
+    GFA2_ALLOC(b.lcp, n);
+    GFA2_ALLOC(b.blocksum, tree_tiles);
+    GFA2_ALLOC(b.S, (size_t)n + 1);
+    GFA2_ALLOC(b.rec, (size_t)cap);
+    GFA2_ALLOC(b.brec, n);
+    GFA2_ALLOC(b.rcount, (size_t)cap);
+    GFA2_ALLOC(b.cell_rec, maxcells);
+    GFA2_ALLOC(b.cell_lvl, maxcells);
+    GFA2_ALLOC(b.level_list, maxcells);
+    GFA2_ALLOC(b.bbox_part, kPartBlocks);
+    b.nwalk_part = walk_blocks(n, h->p.group);
+    GFA2_ALLOC(b.walk_part, b.nwalk_part);
+    GFA2_ALLOC(b.st, 1);
+    if (h->capture_rep) GFA2_ALLOC(b.rep, n);
+#undef GFA2_ALLOC
+    b.nlight = nlight;
+    b.nmedium = (int)medium.size();
+    b.nheavy = (int)heavy.size();
+    GFA2_TRY(hipMemcpyAsync(b.rowptr, rp32.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, h->stream));
+    if (nnz > 0)
+        GFA2_TRY(hipMemcpyAsync(b.col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, h->stream));
+    GFA2_TRY(hipMemcpyAsync(b.mass, m.data(), sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+    GFA2_TRY(hipMemcpyAsync(b.node_lists, lists.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+    GFA2_TRY(hipMemsetAsync(b.pos, 0, sizeof(float2) * n, h->stream));
+    h->edges = nnz / 2;
+    h->has_graph = true;
+    h->has_pos = false;
+    return reset_state(h);
+}
+
+int gfa2_set_graph_coo(gfa2_handle h, int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                       const float* mass) {
+    if (!h || n < 1 || m < 0 || (m > 0 && (!src || !dst))) return n < 1 ? GFA2_ERR_INVALID_GRAPH : GFA2_ERR_INVALID_ARG;
+    std::vector<int64_t> rowptr((size_t)n + 1, 0);
+    for (int64_t e = 0; e < m; ++e) {
+        if (src[e] < 0 || src[e] >= n || dst[e] < 0 || dst[e] >= n || src[e] == dst[e]) return GFA2_ERR_INVALID_GRAPH;
+        rowptr[src[e] + 1]++;
+        rowptr[dst[e] + 1]++;
+    }
+    for (int v = 0; v < n; ++v) rowptr[v + 1] += rowptr[v];
+    std::vector<int64_t> cur(rowptr.begin(), rowptr.end() - 1);
+    std::vector<int32_t> col((size_t)(2 * m));
+    for (int64_t e = 0; e < m; ++e) {
+        col[cur[src[e]]++] = dst[e];
+        col[cur[dst[e]]++] = src[e];
+    }
+    return gfa2_set_graph(h, n, rowptr.data(), col.data(), mass);
+}
+
+int gfa2_set_positions(gfa2_handle h, const float* xy) {
+    if (!h || !xy) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    GFA2_TRY(hipMemcpyAsync(h->b.pos, xy, sizeof(float2) * h->b.n, hipMemcpyHostToDevice, h->stream));
+    launch_bbox_partials(h->b, h->stream);
+    GFA2_TRY(hipGetLastError());
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    h->has_pos = true;
+    return GFA2_OK;
+}
+
+int gfa2_get_positions(gfa2_handle h, float* xy) {
+    if (!h || !xy) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    GFA2_TRY(hipMemcpyAsync(xy, h->b.pos, sizeof(float2) * h->b.n, hipMemcpyDeviceToHost, h->stream));
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    return GFA2_OK;
+}
+
+int gfa2_reset(gfa2_handle h) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    return reset_state(h);
+}
+
+int gfa2_step(gfa2_handle h, int32_t iters) {
+    if (!h || iters < 0) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    if (!h->has_pos) return GFA2_ERR_INVALID_ARG;
+    GFA2_TRY(hipSetDevice(h->device));
+    if (h->timing) {
+        const size_t need = (size_t)(h->pending + iters) * (kGroups + 1);
+        while (h->events.size() < need) {
+            hipEvent_t e;
+            GFA2_TRY(hipEventCreate(&e));
+            h->events.push_back(e);
+        }
+        for (int i = 0; i < iters; ++i) {
+            enqueue_iteration(h, &h->events[(size_t)(h->pending) * (kGroups + 1)]);
+            h->pending++;
+        }
+        GFA2_TRY(hipGetLastError());
+        return GFA2_OK;
+    }
+    if (h->use_graph) {
+        if (!h->graph_valid) {
+            GFA2_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_iteration(h, nullptr);
+            hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
+            if (e != hipSuccess) return hip_status(e);
+            GFA2_TRY(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+            h->graph_valid = true;
+        }
+        for (int i = 0; i < iters; ++i) GFA2_TRY(hipGraphLaunch(h->gexec, h->stream));
+        return GFA2_OK;
+    }
+    for (int i = 0; i < iters; ++i) enqueue_iteration(h, nullptr);
+    GFA2_TRY(hipGetLastError());
+    return GFA2_OK;
+}
+
+int gfa2_sync(gfa2_handle h) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_OK;
+    GFA2_TRY(hipSetDevice(h->device));
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    int rc = drain_timing(h);
+    if (rc != GFA2_OK) return rc;
+    DevState st;
+    rc = read_state(h, &st);
+    if (rc != GFA2_OK) return rc;
+    if (st.err & kErrTreeOverflow) return GFA2_ERR_TREE_OVERFLOW;
+    return GFA2_OK;
+}
+
+int gfa2_get_forces(gfa2_handle h, int32_t kind, float* xy) {
+    if (!h || !xy) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    const float2* src = nullptr;
+    if (kind == GFA2_FORCE_TOTAL) src = h->b.f;
+    else if (kind == GFA2_FORCE_REPULSION) src = h->b.rep;
+    else if (kind == GFA2_FORCE_ATTRACTION_GRAVITY) src = h->b.fga;
+    else return GFA2_ERR_INVALID_ARG;
+    if (!src) return GFA2_ERR_INVALID_ARG;
+    GFA2_TRY(hipMemcpyAsync(xy, src, sizeof(float2) * h->b.n, hipMemcpyDeviceToHost, h->stream));
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    return GFA2_OK;
+}
+
+int gfa2_set_capture(gfa2_handle h, int32_t on) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    h->capture_rep = on != 0;
+    if (h->has_graph && h->capture_rep && !h->b.rep) {
+        GFA2_TRY(hipSetDevice(h->device));
+        int st = dalloc(h, &h->b.rep, h->b.n);
+        if (st != GFA2_OK) return st;
+        GFA2_TRY(hipMemsetAsync(h->b.rep, 0, sizeof(float2) * h->b.n, h->stream));
+    }
+    if (h->has_graph && !h->capture_rep) h->b.rep = nullptr;
+    h->graph_valid = false;
+    if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+    return GFA2_OK;
+}
+
+int gfa2_get_tree(gfa2_handle h, int32_t cap, int32_t* ncells, int32_t* level, int32_t* count, float* mass,
+                  float* xy, int32_t* perm) {
+    if (!h || cap < 0) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    DevState st;
+    int rc = read_state(h, &st);
+    if (rc != GFA2_OK) return rc;
+    if (st.err & kErrTreeOverflow) return GFA2_ERR_TREE_OVERFLOW;
+    const int nrec = st.nrecords;
+    std::vector<float4> rec((size_t)std::max(nrec, 1));
+    std::vector<int> rcount((size_t)std::max(nrec, 1));
+    if (nrec > 0) {
+        GFA2_TRY(hipMemcpyAsync(rec.data(), h->b.rec, sizeof(float4) * nrec, hipMemcpyDeviceToHost, h->stream));
+        GFA2_TRY(hipMemcpyAsync(rcount.data(), h->b.rcount, sizeof(int) * nrec, hipMemcpyDeviceToHost, h->stream));
+    }
+    if (perm)
+        GFA2_TRY(hipMemcpyAsync(perm, h->b.sort.vals[0], sizeof(int) * h->b.n, hipMemcpyDeviceToHost, h->stream));
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    int c = 0;
+    for (int r = 0; r < nrec; ++r) {
+        int info;
+        memcpy(&info, &rec[r].w, sizeof(int));
+        if (info == -1) continue;
+        if (c < cap) {
+            if (level) level[c] = r == 0 ? 0 : (info & 31) + 1;
+            if (count) count[c] = rcount[r];
+            if (mass) mass[c] = rec[r].z;
+            if (xy) { xy[2 * c] = rec[r].x; xy[2 * c + 1] = rec[r].y; }
+        }
+        ++c;
+    }
+    if (ncells) *ncells = c;
+    return GFA2_OK;
+}
+
+int gfa2_get_stats(gfa2_handle h, gfa2_stats* s) {
+    if (!h || !s) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    DevState st;
+    int rc = read_state(h, &st);
+    if (rc != GFA2_OK) return rc;
+    s->iteration = st.iteration;
+    s->nodes = h->b.n;
+    s->edges = h->edges;
+    s->global_speed = st.global_speed;
+    s->speed_efficiency = st.speed_efficiency;
+    s->swing = st.swing;
+    s->traction = st.traction;
+    s->interactions = (int64_t)st.interactions;
+    s->cells = st.ncells;
+    s->max_level = st.max_level;
+    s->root_x = st.rootx;
+    s->root_y = st.rooty;
+    s->radius = st.radius;
+    s->error_flags = st.err;
+    return GFA2_OK;
+}
+
+int gfa2_set_timing(gfa2_handle h, int32_t on) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    if (h->pending) {
+        int rc = drain_timing(h);
+        if (rc != GFA2_OK) return rc;
+    }
+    h->timing = on != 0;
+    for (double& a : h->acc_ms) a = 0.0;
+    h->timed_iters = 0;
+    return GFA2_OK;
+}
+
+int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** names, double* ms) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    int rc = drain_timing(h);
+    if (rc != GFA2_OK) return rc;
+    if (count) *count = kGroups;
+    for (int g = 0; g < kGroups && g < cap; ++g) {
+        if (names) names[g] = kGroupNames[g];
+        if (ms) ms[g] = h->timed_iters ? h->acc_ms[g] / (double)h->timed_iters : 0.0;
+    }
+    return GFA2_OK;
+}
+
+int gfa2_set_graph_capture(gfa2_handle h, int32_t on) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    h->use_graph = on != 0;
+    h->graph_valid = false;
+    if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+    return GFA2_OK;
+}
+
+int gfa2_destroy(gfa2_handle h) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    free_all(h);
+    for (hipEvent_t e : h->events) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return GFA2_OK;
+}
+
+const char* gfa2_strerror(int code) {
+    switch (code) {
+        case GFA2_OK: return "ok";
+        case GFA2_ERR_NO_DEVICE: return "no HIP device found";
+        case GFA2_ERR_OOM: return "out of device memory";
+        case GFA2_ERR_INVALID_ARG: return "invalid argument";
+        case GFA2_ERR_INVALID_GRAPH: return "invalid graph (empty, malformed CSR, self-loop or id out of range)";
+        case GFA2_ERR_TREE_OVERFLOW: return "quadtree cell capacity exceeded";
+        case GFA2_ERR_HIP: return "HIP runtime error";
+        case GFA2_ERR_NO_GRAPH: return "no graph loaded";
+        case GFA2_ERR_UNSUPPORTED: return "unsupported configuration";
+        default: return "unknown error";
+    }
+}
+
+}  // extern "C"

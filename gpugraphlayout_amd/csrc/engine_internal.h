@@ -1,0 +1,129 @@
+// Internal declarations shared by the engine's HIP translation units.
+//
+// Device data layout of one layout instance (V bodies, E undirected edges):
+//   pos      float2[V]   positions, caller's node order
+//   mass     float[V]    deg+1 unless given
+//   rowptr   int[V+1], col int[2E]   symmetric CSR
+//   fga      float2[V]   gravity + attraction of the current iteration
+//   f        float2[V]   total force of the current iteration
+//   fprev    float2[V]   total force of the previous iteration (F')
+//   keys     u64[V] x2, vals u32[V] x2   quadtree path keys and body ids (sort ping-pong)
+//   lcp      i8[V]       common-prefix length (levels) of sorted neighbours i, i+1
+//   S        int[V+1]    exclusive prefix of cells opened at each sorted position
+//   rec      float4[V+C] the quadtree in pre-order: {x, y, mass, info}
+//            info == -1: a body; otherwise a cell, (skip << 5) | (level - 1) as unsigned,
+//            skip being the number of records in its subtree (the walk jumps by it)
+//   brec     int[V]      record index of the body at sorted position k
+//   rcount   int[V+C]    body count of each cell record
+// Tree levels: the root is level 0; a key holds 32 levels of 2-bit quadrant digits, so
+// cells exist down to level 32. Bodies that share all 32 digits share one level-32 cell.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace gfa2 {
+
+constexpr int kKeyLevels = 32;          // quadrant digits per key
+constexpr int kMaxLevels = kKeyLevels + 1;
+constexpr int kInfoLevelBits = 5;
+constexpr int kMaxRecords = (1 << 27) - 2;  // skip must fit in 27 bits of info
+constexpr int kPartBlocks = 1024;       // bounding-box partials (displacement grid cap)
+
+// Error flags raised on the device, read back at synchronisation points.
+constexpr int kErrTreeOverflow = 1;
+
+// Per-instance scalar state, resident in device memory.
+struct DevState {
+    float rootx, rooty, radius, pad0;
+    float global_speed, speed_efficiency, jitter_tolerance, k_s_max;
+    double swing, traction;
+    unsigned long long interactions;
+    int ncells;
+    int nrecords;
+    int max_level;      // deepest cell level
+    int err;
+    int iteration;
+    int nbbox;          // number of valid bounding-box partials
+    int pad1[2];
+    int level_count[kMaxLevels + 1];
+    int level_offset[kMaxLevels + 2];
+    int level_cursor[kMaxLevels + 1];
+};
+
+struct WalkPart {
+    float swing, traction;
+    unsigned int interactions, pad;
+};
+
+struct SortBuffers {
+    unsigned long long* keys[2];
+    unsigned int* vals[2];
+    unsigned int* hist;      // 256 x tiles
+    unsigned int* rowtot;    // 256
+};
+
+// FA2 scalar parameters (src/RPForceAtlas2.cpp:28-55 defaults).
+struct Params {
+    int strong_gravity;
+    float k_g, k_r;
+    float itolsq, epssq;
+    int group;              // walk vote group: 32 or 64 lanes
+};
+
+// All device buffers of one instance.
+struct Bufs {
+    int n;                  // V
+    long long nnz;          // 2E
+    int cap_records;        // capacity of rec / rcount (V + max cells)
+    float2* pos;
+    float* mass;
+    int* rowptr;
+    int* col;
+    int* node_lists;        // light | medium | heavy node ids
+    int nlight, nmedium, nheavy;
+    float2* fga;
+    float2* f;
+    float2* fprev;
+    float2* rep;            // repulsion capture (debug), may be null
+    SortBuffers sort;
+    signed char* lcp;
+    int* blocksum;
+    int* S;
+    float4* rec;
+    int* brec;
+    int* rcount;
+    int* cell_rec;
+    unsigned char* cell_lvl;
+    int* level_list;
+    float4* bbox_part;
+    WalkPart* walk_part;
+    int nwalk_part;
+    DevState* st;
+};
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kTreeThreads = 256;
+constexpr int kTreeItems = 8;
+constexpr int kTreeTile = kTreeThreads * kTreeItems;
+
+// Radix sort of (key, value) pairs on 64-bit keys, ascending and stable. Sorts
+// buf.keys[0]/vals[0] in place (via buf.keys[1]/vals[1]).
+void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s);
+
+// Launch groups of one iteration, in order.
+void launch_attraction(const Bufs& b, const Params& p, hipStream_t s);
+void launch_bbox_partials(const Bufs& b, hipStream_t s);
+void launch_keys(const Bufs& b, hipStream_t s);
+void launch_tree(Bufs& b, hipStream_t s);              // after the sort: cells + summaries
+void launch_walk(const Bufs& b, const Params& p, hipStream_t s);
+void launch_speed(const Bufs& b, hipStream_t s);
+void launch_displacement(const Bufs& b, hipStream_t s);
+
+// Walk grid sizes (partials count depends on them).
+int walk_blocks(int n, int group);
+int displacement_blocks(int n);
+
+}  // namespace gfa2

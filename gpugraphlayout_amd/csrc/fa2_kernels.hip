@@ -1,0 +1,369 @@
+// ForceAtlas2 force, speed and displacement kernels for gfx950 (FP32 VALU throughout).
+#include "engine_internal.h"
+
+namespace gfa2 {
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// Gravity + attraction, one pass over the symmetric CSR.
+// Reference: GravityKernel (src/RPFA2Kernels.cu:35-73) and AttractiveForceKernel
+// (:75-109). The reference adds each edge's p_t - p_s to F_s and its negation to F_t with
+// four float atomics per edge, in no fixed order; here each node gathers
+// sum_j (p_j - p_i) over its own row, which holds exactly the same terms, in a fixed
+// order and without atomics. Nodes are pre-sorted into degree classes:
+//   light (deg <= 32): one thread per node, sequential sum
+//   medium (deg <= 2048): one wave per node, lane-strided sums + butterfly reduction
+//   heavy: one 256-thread block per node, thread-strided sums + LDS tree reduction
+// ---------------------------------------------------------------------------------------
+constexpr int kLightMax = 32;
+
+__device__ __forceinline__ float2 gravity(float2 p, float m, int strong, float k_g) {
+    float fg;
+    if (strong) fg = k_g * m;
+    else if (p.x != 0.0f || p.y != 0.0f) fg = k_g * m * rsqrtf(p.x * p.x + p.y * p.y);
+    else fg = 0.0f;
+    return make_float2(-p.x * fg, -p.y * fg);
+}
+
+__global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos, const float* __restrict__ mass,
+                                                 const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                 const int* __restrict__ lists, int nlight, int nmedium,
+                                                 int nheavy, int light_blocks, int medium_blocks,
+                                                 int strong, float k_g, float2* __restrict__ fga) {
+    __shared__ float2 red[256];
+    const int tid = threadIdx.x;
+    int blk = blockIdx.x;
+    if (blk < light_blocks) {
+        const int k = blk * 256 + tid;
+        if (k >= nlight) return;
+        const int v = lists[k];
+        const float2 p = pos[v];
+        float ax = 0.0f, ay = 0.0f;
+        const int e1 = rowptr[v + 1];
+        for (int e = rowptr[v]; e < e1; ++e) {
+            const float2 q = pos[col[e]];
+            ax += q.x - p.x;
+            ay += q.y - p.y;
+        }
+        const float2 g = gravity(p, mass[v], strong, k_g);
+        fga[v] = make_float2(g.x + ax, g.y + ay);
+        return;
+    }
+    blk -= light_blocks;
+    if (blk < medium_blocks) {
+        const int k = blk * 4 + (tid >> 6);
+        if (k >= nmedium) return;
+        const int lane = tid & 63;
+        const int v = lists[nlight + k];
+        const float2 p = pos[v];
+        float ax = 0.0f, ay = 0.0f;
+        const int e1 = rowptr[v + 1];
+        for (int e = rowptr[v] + lane; e < e1; e += 64) {
+            const float2 q = pos[col[e]];
+            ax += q.x - p.x;
+            ay += q.y - p.y;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            ax += __shfl_xor(ax, o, 64);
+            ay += __shfl_xor(ay, o, 64);
+        }
+        if (lane == 0) {
+            const float2 g = gravity(p, mass[v], strong, k_g);
+            fga[v] = make_float2(g.x + ax, g.y + ay);
+        }
+        return;
+    }
+    blk -= medium_blocks;
+    if (blk >= nheavy) return;
+    const int v = lists[nlight + nmedium + blk];
+    const float2 p = pos[v];
+    float ax = 0.0f, ay = 0.0f;
+    const int e1 = rowptr[v + 1];
+    for (int e = rowptr[v] + tid; e < e1; e += 256) {
+        const float2 q = pos[col[e]];
+        ax += q.x - p.x;
+        ay += q.y - p.y;
+    }
+    red[tid] = make_float2(ax, ay);
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            float2 a = red[tid], b = red[tid + o];
+            red[tid] = make_float2(a.x + b.x, a.y + b.y);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const float2 g = gravity(p, mass[v], strong, k_g);
+        fga[v] = make_float2(g.x + red[0].x, g.y + red[0].y);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Barnes-Hut repulsion walk. Reference: ForceCalculationKernel, src/RPBHKernels.cu:566-693.
+//
+// Groups of G consecutive bodies of the sorted (in-order) permutation walk the tree
+// together. G = 32 reproduces the reference's 32-lane warp vote on wave64 by giving each
+// half-wave its own walk; G = 64 votes over the whole wave. The tree is the pre-order
+// record array, so the walk needs no stack: from record r, a body is always taken
+// directly (r+1); a cell is taken as a whole when every active lane of the group has
+// dx^2+dy^2+epssq >= dq[level-1] (r += skip), and otherwise opened (r+1). That visits
+// children in quadrant order and accumulates in the reference's DFS order.
+// dq[d] = (2*radius)^2*itolsq*0.25^d + epssq (src/RPBHKernels.cu:576-590) is formed with
+// ldexp, which is exact for the power-of-4 scaling.
+// The term k_r*m_i*m_n*dx/(d^2+eps^2) uses a hardware reciprocal (the reference divides);
+// the difference is an ulp or two per term.
+// The epilogue adds gravity + attraction, writes the total force, and forms this
+// group's swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161) so the speed
+// step reads only per-block partials.
+// ---------------------------------------------------------------------------------------
+template <int G>
+__global__ __launch_bounds__(256) void k_walk(const float4* __restrict__ rec, const int* __restrict__ brec,
+                                              const unsigned int* __restrict__ perm, int n,
+                                              const float2* __restrict__ fga, const float2* __restrict__ fprev,
+                                              float2* __restrict__ f, float2* __restrict__ rep,
+                                              WalkPart* __restrict__ part, float k_r, float itolsq, float epssq,
+                                              const DevState* __restrict__ st) {
+    __shared__ float red_s[4], red_t[4];
+    __shared__ unsigned int red_i[4];
+    if (st->err != 0) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    // XCD-aware block order: consecutive sorted ranges stay on one XCD's L2.
+    const int nb = gridDim.x;
+    const int xcd = blockIdx.x & 7;
+    const int per = (nb + 7) >> 3;
+    const int full = nb & 7;   // number of XCDs that get 'per' blocks when nb % 8 != 0
+    int lb;
+    if (full == 0) lb = xcd * per + (blockIdx.x >> 3);
+    else lb = (xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1)) + (blockIdx.x >> 3);
+    const int k = lb * 256 + tid;
+    const bool active = k < n;
+
+    const int nrec = st->nrecords;
+    const float two_r = st->radius * 2;
+    const float base = two_r * two_r * itolsq;
+
+    float px = 0.0f, py = 0.0f, mi = 0.0f, kmi = 0.0f;
+    unsigned int id = 0;
+    if (active) {
+        const float4 me = rec[brec[k]];
+        px = me.x;
+        py = me.y;
+        mi = me.z;
+        kmi = k_r * mi;
+        id = perm[k];
+    }
+    float ax = 0.0f, ay = 0.0f;
+    unsigned int inter = 0;
+    // Every lane of a group walks, active or not, so the group's record index stays uniform.
+    const int gbase = (G == 32) ? (lane & 32) : 0;
+    const bool group_live = __ballot(active) != 0ull;
+    int r = 1;
+    if (group_live) {
+        while (r < nrec) {
+            const float4 c = rec[r];
+            const unsigned int info = __float_as_uint(c.w);
+            const float dx = px - c.x;
+            const float dy = py - c.y;
+            const float d2 = dx * dx + dy * dy + epssq;
+            const bool body = info == 0xffffffffu;
+            const float thr = ldexpf(base, -2 * (int)(info & 31u)) + epssq;
+            const unsigned long long fails = __ballot(active && !body && !(d2 >= thr));
+            const unsigned int gf = (G == 32) ? (unsigned int)(fails >> gbase) : (fails != 0ull ? 1u : 0u);
+            const bool accept = body || gf == 0u;
+            if (accept) {
+                const float w = kmi * c.z * __builtin_amdgcn_rcpf(d2);
+                ax = fmaf(dx, w, ax);
+                ay = fmaf(dy, w, ay);
+                inter += active ? 1u : 0u;
+                r += body ? 1 : max((int)(info >> kInfoLevelBits), 1);
+            } else {
+                r += 1;
+            }
+        }
+    }
+    float sw = 0.0f, tr = 0.0f;
+    if (active) {
+        const float2 a = fga[id];
+        const float2 F = make_float2(a.x + ax, a.y + ay);
+        f[id] = F;
+        if (rep) rep[id] = make_float2(ax, ay);
+        const float2 P = fprev[id];
+        float dx = F.x - P.x, dy = F.y - P.y;
+        sw = mi * sqrtf(dx * dx + dy * dy);
+        dx = F.x + P.x;
+        dy = F.y + P.y;
+        tr = mi * (sqrtf(dx * dx + dy * dy) * 0.5f);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sw += __shfl_xor(sw, o, 64);
+        tr += __shfl_xor(tr, o, 64);
+        inter += __shfl_xor(inter, o, 64);
+    }
+    if (lane == 0) {
+        red_s[wave] = sw;
+        red_t[wave] = tr;
+        red_i[wave] = inter;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        WalkPart wp;
+        wp.swing = (red_s[0] + red_s[1]) + (red_s[2] + red_s[3]);
+        wp.traction = (red_t[0] + red_t[1]) + (red_t[2] + red_t[3]);
+        wp.interactions = red_i[0] + red_i[1] + red_i[2] + red_i[3];
+        wp.pad = 0;
+        part[lb] = wp;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Speed update. Reference: SpeedKernel's last block, src/RPFA2Kernels.cu:167-219 (Gephi's
+// adaptive speed). The block partials are summed in double in a fixed order; the update
+// itself is the reference's float arithmetic, including its double literals.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_speed(const WalkPart* __restrict__ part, int np, int n, DevState* st) {
+    __shared__ double rs[1024], rt[1024];
+    __shared__ unsigned long long ri[1024];
+    if (st->err != 0) return;
+    const int tid = threadIdx.x;
+    double s = 0.0, t = 0.0;
+    unsigned long long it = 0;
+    for (int k = tid; k < np; k += 1024) {
+        WalkPart p = part[k];
+        s += (double)p.swing;
+        t += (double)p.traction;
+        it += p.interactions;
+    }
+    rs[tid] = s;
+    rt[tid] = t;
+    ri[tid] = it;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (tid < o) {
+            rs[tid] += rs[tid + o];
+            rt[tid] += rt[tid + o];
+            ri[tid] += ri[tid + o];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double S = rs[0], T = rt[0];
+        st->swing = S;
+        st->traction = T;
+        st->interactions = ri[0];
+        const float swg = (float)S, etra = (float)T;
+        float gs = st->global_speed, se = st->speed_efficiency;
+        const float jitter = st->jitter_tolerance, ksmax = st->k_s_max;
+        const float eojt = (float)(0.05 * sqrtf((float)n));
+        const float minjt = sqrtf(eojt);
+        float jt = jitter * fmaxf(minjt, fminf(ksmax, eojt * etra / powf((float)n, 2.0f)));
+        const float min_se = (float)0.05;
+        if (swg / etra > 2.0) {
+            if (se > min_se) se = (float)(se * 0.5);
+            jt = fmaxf(jt, jitter);
+        }
+        const float target = jt * se * etra / swg;
+        if (swg > jt * etra) {
+            if (se > min_se) se = (float)(se * 0.7);
+        } else if (gs < 1000) {
+            se = (float)(se * 1.3);
+        }
+        gs += fminf(target - gs, (float)0.5 * gs);
+        st->global_speed = gs;
+        st->speed_efficiency = se;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Displacement. Reference: DisplacementKernel, src/RPFA2Kernels.cu:223-251:
+// factor = gs/(1.0 + sqrtf(gs*swg)) (double, because of the literal), p += F*factor,
+// F' = F. Also emits bounding-box partials of the new positions for the next tree.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_displace(float2* __restrict__ pos, const float2* __restrict__ f,
+                                                  float2* __restrict__ fprev, int n, float4* __restrict__ part,
+                                                  DevState* st) {
+    __shared__ float4 red[256];
+    const int tid = threadIdx.x;
+    const bool live = st->err == 0;
+    const float gs = st->global_speed;
+    float4 acc = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
+    for (int i = blockIdx.x * 256 + tid; i < n; i += gridDim.x * 256) {
+        float2 p = pos[i];
+        if (live) {
+            const float2 F = f[i];
+            const float2 P = fprev[i];
+            const float dx = F.x - P.x, dy = F.y - P.y;
+            const float swg = sqrtf(dx * dx + dy * dy);
+            const float factor = (float)((double)gs / (1.0 + (double)sqrtf(gs * swg)));
+            p.x = fmaf(F.x, factor, p.x);
+            p.y = fmaf(F.y, factor, p.y);
+            pos[i] = p;
+            fprev[i] = F;
+        }
+        acc.x = fminf(acc.x, p.x);
+        acc.y = fminf(acc.y, p.y);
+        acc.z = fmaxf(acc.z, p.x);
+        acc.w = fmaxf(acc.w, p.y);
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            float4 v = red[tid + o];
+            acc.x = fminf(acc.x, v.x);
+            acc.y = fminf(acc.y, v.y);
+            acc.z = fmaxf(acc.z, v.z);
+            acc.w = fmaxf(acc.w, v.w);
+            red[tid] = acc;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        part[blockIdx.x] = red[0];
+        if (blockIdx.x == 0) {
+            st->nbbox = gridDim.x;
+            if (live) st->iteration += 1;
+        }
+    }
+}
+
+}  // namespace
+
+int walk_blocks(int n, int group) {
+    (void)group;
+    int b = (n + 255) / 256;
+    return b < 1 ? 1 : b;
+}
+
+void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
+    const int lb = (b.nlight + 255) / 256;
+    const int mb = (b.nmedium + 3) / 4;
+    const int total = lb + mb + b.nheavy;
+    if (total == 0) return;
+    k_attract<<<total, 256, 0, s>>>(b.pos, b.mass, b.rowptr, b.col, b.node_lists, b.nlight, b.nmedium, b.nheavy,
+                                    lb, mb, p.strong_gravity, p.k_g, b.fga);
+}
+
+void launch_walk(const Bufs& b, const Params& p, hipStream_t s) {
+    const int blocks = walk_blocks(b.n, p.group);
+    if (p.group == 64)
+        k_walk<64><<<blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.n, b.fga, b.fprev, b.f, b.rep,
+                                          b.walk_part, p.k_r, p.itolsq, p.epssq, b.st);
+    else
+        k_walk<32><<<blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.n, b.fga, b.fprev, b.f, b.rep,
+                                          b.walk_part, p.k_r, p.itolsq, p.epssq, b.st);
+}
+
+void launch_speed(const Bufs& b, hipStream_t s) {
+    k_speed<<<1, 1024, 0, s>>>(b.walk_part, walk_blocks(b.n, 32), b.n, b.st);
+}
+
+void launch_displacement(const Bufs& b, hipStream_t s) {
+    k_displace<<<displacement_blocks(b.n), 256, 0, s>>>(b.pos, b.f, b.fprev, b.n, b.bbox_part, b.st);
+}
+
+}  // namespace gfa2

@@ -1,0 +1,145 @@
+/*
+ * gfa2.h — C ABI of the MI355X ForceAtlas2 engine (libgfa2.so).
+ *
+ * The reference (govertb/GPUGraphLayout) has no C layer: its host class CUDAForceAtlas2
+ * launches kernels directly. Each entry point below replaces a piece of that class or
+ * of the ForceAtlas2/LayoutAlgorithm interface it implements; the comment on each names
+ * the reference lines. Plain pointers and sizes only; every call returns an int status
+ * (GFA2_OK or a negative GFA2_ERR_*). The library never retains caller pointers. A
+ * handle owns one HIP stream and all its device memory; handles are independent but a
+ * single handle is not thread-safe.
+ */
+#ifndef GFA2_H
+#define GFA2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFA2_OK 0
+#define GFA2_ERR_NO_DEVICE (-1)      /* src/RPGPUForceAtlas2.cu:42-48 exits instead */
+#define GFA2_ERR_OOM (-2)            /* cudaMalloc failure, src/RPGPUForceAtlas2.cu:120-149 */
+#define GFA2_ERR_INVALID_ARG (-3)
+#define GFA2_ERR_INVALID_GRAPH (-4)  /* empty graph, bad CSR, self-loops, ids out of range */
+#define GFA2_ERR_TREE_OVERFLOW (-5)  /* the reference overflows silently, src/RPBHKernels.cu:285-286 */
+#define GFA2_ERR_HIP (-6)            /* any other HIP runtime error, src/RPCommon.hpp:33-41 exits */
+#define GFA2_ERR_NO_GRAPH (-7)       /* step/positions before set_graph */
+#define GFA2_ERR_UNSUPPORTED (-8)    /* e.g. exact mode: src/graph_viewer.cpp:93-97 */
+
+typedef struct gfa2_engine* gfa2_handle;
+
+/* FA2 parameters. Defaults (gfa2_default_params) are the reference's
+ * (src/RPForceAtlas2.cpp:28-55). */
+typedef struct gfa2_params {
+    int32_t use_barneshut;      /* must be 1 (the GPU path is Barnes-Hut only) */
+    int32_t strong_gravity;     /* sg / wg, src/graph_viewer.cpp:62 */
+    float gravity;              /* k_g, ForceAtlas2::setGravity, src/RPForceAtlas2.cpp:69-72 */
+    float scale;                /* k_r, ForceAtlas2::setScale, src/RPForceAtlas2.cpp:64-67 */
+    float theta;                /* 1.0; itolsq = 1/theta^2 */
+    float epssq;                /* 0.05*0.05 */
+    float jitter_tolerance;     /* 1.0 */
+    float k_s_max;              /* 10.0 */
+    float global_speed;         /* initial speed, 1.0 */
+    float speed_efficiency;     /* initial efficiency, 1.0 */
+    int32_t group_size;         /* walk vote group: 32 = the reference's warp vote; 64 = wave64 */
+    int32_t cell_factor;        /* cell capacity = cell_factor * V (+ slack); 0 = default 3 */
+} gfa2_params;
+
+/* Per-iteration statistics (no counterpart in the reference, which reads none back). */
+typedef struct gfa2_stats {
+    int32_t iteration;          /* completed iterations (ForceAtlas2::iteration) */
+    int32_t nodes;
+    int64_t edges;              /* undirected */
+    float global_speed;
+    float speed_efficiency;
+    double swing;               /* S of the last iteration */
+    double traction;            /* T of the last iteration */
+    int64_t interactions;       /* body-node interactions of the last walk */
+    int32_t cells;              /* quadtree cells of the last tree */
+    int32_t max_level;          /* deepest cell level (root 0); max body depth = max_level + 1 */
+    float root_x, root_y, radius;
+    int32_t error_flags;
+} gfa2_stats;
+
+/* Force components for gfa2_get_forces. */
+#define GFA2_FORCE_TOTAL 0          /* F of the last iteration (F' after it) */
+#define GFA2_FORCE_REPULSION 1      /* Barnes-Hut part; needs gfa2_set_capture(h, 1) */
+#define GFA2_FORCE_ATTRACTION_GRAVITY 2
+
+void gfa2_default_params(gfa2_params* p);
+
+/* Replaces the constructor's device setup, src/RPGPUForceAtlas2.cu:37-49, :86-117 (device
+ * query, buffer sizing). device: HIP ordinal. */
+int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out);
+
+/* Replaces host staging + H2D copies of masses and edges, src/RPGPUForceAtlas2.cu:51-84,
+ * :152-161 and sendGraphToGPU :262-268. Symmetric CSR: rowptr[n+1] (int64), col[rowptr[n]]
+ * holds every undirected edge in both rows. mass may be NULL: then deg+1
+ * (ForceAtlas2::mass, src/RPForceAtlas2.cpp:74-77). Resets the FA2 state. */
+int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_t* col, const float* mass);
+
+/* Same, from the reference's COO list (each undirected edge once, as
+ * src/RPGPUForceAtlas2.cu:73-84 stages it). */
+int gfa2_set_graph_coo(gfa2_handle h, int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                       const float* mass);
+
+/* H2D positions (interleaved x,y; n pairs). The reference's private sendLayoutToGPU,
+ * src/RPGPUForceAtlas2.cu:256-260, made public. Does not reset F' or the speed state. */
+int gfa2_set_positions(gfa2_handle h, const float* xy);
+
+/* D2H positions: retrieveLayoutFromGPU, src/RPGPUForceAtlas2.cu:250-254 (the copy half of
+ * sync_layout, :270-278). Waits for queued iterations. */
+int gfa2_get_positions(gfa2_handle h, float* xy);
+
+/* Fresh FA2 state: F' = 0, speed/efficiency from params, iteration 0. The reference only
+ * has this at construction (src/RPGPUForceAtlas2.cu:63-71, src/RPFA2Kernels.cu:28-33). */
+int gfa2_reset(gfa2_handle h);
+
+/* Enqueue iters iterations: CUDAForceAtlas2::doStep, src/RPGPUForceAtlas2.cu:206-248, and
+ * ForceAtlas2::doSteps, src/RPForceAtlas2.cpp:59-62. Asynchronous: no host sync per
+ * iteration. */
+int gfa2_step(gfa2_handle h, int32_t iters);
+
+/* Wait for queued work; reports device-side errors (tree overflow). The reference's
+ * cudaDeviceSynchronize at src/RPGPUForceAtlas2.cu:246. */
+int gfa2_sync(gfa2_handle h);
+
+/* Force vectors of the last iteration (interleaved x,y). */
+int gfa2_get_forces(gfa2_handle h, int32_t kind, float* xy);
+
+/* Keep the repulsion of each iteration in a separate buffer (debug / parity). */
+int gfa2_set_capture(gfa2_handle h, int32_t on);
+
+/* The last quadtree, cells in pre-order (canonical: independent of build order):
+ * level (root 0), body count, mass, centroid (x,y). perm: the in-order body permutation
+ * (the reference's sortd, src/RPBHKernels.cu:513-559). Any output pointer may be NULL.
+ * *ncells receives the cell count; cap bounds what is written. */
+int gfa2_get_tree(gfa2_handle h, int32_t cap, int32_t* ncells, int32_t* level, int32_t* count, float* mass,
+                  float* xy, int32_t* perm);
+
+int gfa2_get_stats(gfa2_handle h, gfa2_stats* s);
+
+/* Per-kernel-group timing with HIP events on the handle's stream (no counterpart in the
+ * reference). When on, every iteration records events between launch groups. */
+int gfa2_set_timing(gfa2_handle h, int32_t on);
+/* Average milliseconds per iteration of each group since timing was enabled:
+ * names[i] (static strings), ms[i]; *count receives the number of groups (<= cap). */
+int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** names, double* ms);
+
+/* Graph capture of the whole iteration (hipGraph), replayed per iteration. */
+int gfa2_set_graph_capture(gfa2_handle h, int32_t on);
+
+int gfa2_destroy(gfa2_handle h);
+
+const char* gfa2_strerror(int code);
+
+/* Device facts: compute units, wavefront size, total memory (bytes). */
+int gfa2_device_info(int device, int32_t* cus, int32_t* wave, int64_t* mem_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GFA2_H */

@@ -1,0 +1,176 @@
+"""Engine (HIP, through the C ABI) against the oracle, on the same seeded inputs.
+
+Bars: tree shape, cell counts, cell masses and the body permutation bit-exact; cell
+centroids within 1e-6 of the root box size; Barnes-Hut repulsion within 1e-4 relative
+L2 of the oracle's reference-semantics walk at the same theta and group size; gravity +
+attraction within 1e-5 relative L2; one full iteration's positions within 1e-4 of the
+box size and speed state within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from gpugraphlayout_amd import Engine, GFA2_FORCE_ATTRACTION_GRAVITY, GFA2_FORCE_REPULSION, GFA2_FORCE_TOTAL
+from gpugraphlayout_amd import graphs
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.float32(0.05 * 0.05)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def make_case(kind, seed):
+    if kind == "ba":
+        s, t = graphs.barabasi_albert(3000, 4, seed)
+    elif kind == "rmat":
+        s, t = graphs.rmat(13, 40000, seed=seed)
+    elif kind == "sbm":
+        s, t, _ = graphs.sbm(4000, 8, 2.0, seed)
+    elif kind == "tiny":
+        s, t = np.array([0, 1, 2]), np.array([1, 2, 0])
+    elif kind == "pair":
+        s, t = np.array([0]), np.array([1])
+    else:
+        raise ValueError(kind)
+    g = graphs.edges_to_graph(s, t)
+    pos = graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), seed + 17)
+    return g, pos
+
+
+def engine_first_step(g, pos, group=32, strong=False, k_g=1.0, k_r=10.0):
+    eng = Engine(g.n, g.src, g.dst, group_size=group, strong_gravity=int(strong), gravity=k_g, scale=k_r)
+    eng.set_capture(True)
+    eng.set_positions(pos)
+    eng.step(1)
+    eng.sync()
+    out = {
+        "tree": eng.tree(),
+        "rep": eng.forces(GFA2_FORCE_REPULSION),
+        "fga": eng.forces(GFA2_FORCE_ATTRACTION_GRAVITY),
+        "f": eng.forces(GFA2_FORCE_TOTAL),
+        "pos": eng.positions(),
+        "stats": eng.stats(),
+    }
+    eng.close()
+    return out
+
+
+@pytest.mark.parametrize("kind,seed", [("ba", 0), ("rmat", 1), ("sbm", 2), ("tiny", 3), ("pair", 4)])
+def test_tree_bit_exact(oracle, kind, seed):
+    print(f"case tree {kind} seed={seed}", flush=True)
+    g, pos = make_case(kind, seed)
+    out = engine_first_step(g, pos)
+    ref = oracle.tree(pos, g.mass())
+    T = out["tree"]
+    assert len(T["level"]) == len(ref["level"])
+    np.testing.assert_array_equal(T["level"], ref["level"])
+    np.testing.assert_array_equal(T["count"], ref["count"])
+    np.testing.assert_array_equal(T["mass"].view(np.uint32), ref["mass"].view(np.uint32))
+    np.testing.assert_array_equal(T["perm"], ref["perm"])
+    scale = float(ref["root"][2]) * 2
+    assert np.max(np.abs(T["xy"] - ref["xy"])) <= 1e-6 * scale
+    assert out["stats"]["max_level"] + 1 == ref["max_depth"]
+    assert out["stats"]["cells"] == len(ref["level"])
+
+
+@pytest.mark.parametrize("group", [32, 64])
+@pytest.mark.parametrize("kind,seed", [("ba", 5), ("rmat", 6), ("sbm", 7)])
+def test_forces_first_iteration(oracle, kind, seed, group):
+    print(f"case forces {kind} seed={seed} G={group}", flush=True)
+    g, pos = make_case(kind, seed)
+    out = engine_first_step(g, pos, group=group)
+    rep, info = oracle.bh_repulsion(pos, g.mass(), 10.0, group=group)
+    assert rel_l2(out["rep"], rep) <= 1e-4
+    fga = oracle.attraction_coo(pos, g.src, g.dst, oracle.gravity(pos, g.mass(), False, 1.0))
+    assert rel_l2(out["fga"], fga) <= 1e-5
+    assert rel_l2(out["f"], fga + rep) <= 1e-4
+    assert abs(out["stats"]["interactions"] - info["interactions"]) <= 1e-4 * info["interactions"]
+
+
+@pytest.mark.parametrize("strong", [False, True])
+def test_one_iteration_positions(oracle, strong):
+    print(f"case step strong={strong}", flush=True)
+    g, pos = make_case("ba", 11)
+    k_g, k_r = 1.0, 2.0
+    out = engine_first_step(g, pos, strong=strong, k_g=k_g, k_r=k_r)
+    state = np.array([1.0, 1.0, 1.0, 10.0], np.float32)
+    p1, fp1, st1, info = oracle.fa2_step(pos, np.zeros_like(pos), state, g.src, g.dst, g.mass(), strong, k_g, k_r)
+    box = 2 * np.sqrt(g.n)
+    assert np.max(np.abs(out["pos"] - p1)) <= 1e-4 * box
+    assert abs(out["stats"]["swing"] - info["S"]) <= 1e-5 * info["S"]
+    assert abs(out["stats"]["traction"] - info["T"]) <= 1e-5 * info["T"]
+    assert abs(out["stats"]["global_speed"] - st1[0]) <= 1e-5 * abs(st1[0])
+    assert out["stats"]["speed_efficiency"] == pytest.approx(float(st1[1]), rel=1e-6)
+
+
+def test_forces_after_many_iterations(oracle):
+    """Force parity on the engine's own state after 60 iterations (positions downloaded,
+    then the oracle recomputes the repulsion on exactly those positions)."""
+    print("case forces after 60 iterations", flush=True)
+    g, pos = make_case("rmat", 21)
+    eng = Engine(g.n, g.src, g.dst, scale=10.0, gravity=1.0)
+    eng.set_positions(pos)
+    eng.step(60)
+    eng.sync()
+    p60 = eng.positions()
+    eng.set_capture(True)
+    eng.step(1)
+    eng.sync()
+    rep_gpu = eng.forces(GFA2_FORCE_REPULSION)
+    rep, _ = oracle.bh_repulsion(p60, g.mass(), 10.0, group=32)
+    assert rel_l2(rep_gpu, rep) <= 1e-4
+    eng.close()
+
+
+def test_graph_capture_matches_eager():
+    print("case hipGraph vs eager", flush=True)
+    g, pos = make_case("ba", 31)
+    res = []
+    for cap in (False, True):
+        eng = Engine(g.n, g.src, g.dst, scale=2.0)
+        eng.set_graph_capture(cap)
+        eng.set_positions(pos)
+        for _ in range(3):
+            eng.step(5)
+            eng.sync()
+        res.append(eng.positions())
+        eng.close()
+    np.testing.assert_array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+
+
+def test_deterministic():
+    print("case determinism", flush=True)
+    g, pos = make_case("rmat", 41)
+    res = []
+    for _ in range(2):
+        eng = Engine(g.n, g.src, g.dst, scale=10.0)
+        eng.set_positions(pos)
+        eng.step(20)
+        eng.sync()
+        res.append(eng.positions())
+        eng.close()
+    np.testing.assert_array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+
+
+def test_coincident_bodies_are_finite():
+    """Bodies at identical positions (which the reference perturbs in race order) share a
+    deepest-level cell here; forces and positions stay finite and every body is counted."""
+    print("case coincident", flush=True)
+    g, pos = make_case("ba", 51)
+    pos[10] = pos[11] = pos[12] = pos[3]
+    eng = Engine(g.n, g.src, g.dst, scale=2.0)
+    eng.set_positions(pos)
+    eng.set_capture(True)
+    eng.step(3)
+    eng.sync()
+    T = eng.tree()
+    assert T["count"][0] == g.n
+    assert np.all(np.isfinite(eng.positions()))
+    assert np.all(np.isfinite(eng.forces(GFA2_FORCE_REPULSION)))
+    assert sorted(T["perm"].tolist()) == list(range(g.n))
+    eng.close()
