@@ -61,6 +61,7 @@ class Stats(ctypes.Structure):
         ("swing", ctypes.c_double),
         ("traction", ctypes.c_double),
         ("interactions", ctypes.c_int64),
+        ("interactions_total", ctypes.c_int64),
         ("cells", ctypes.c_int32),
         ("max_level", ctypes.c_int32),
         ("root_x", ctypes.c_float),

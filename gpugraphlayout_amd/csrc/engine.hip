@@ -474,6 +474,7 @@ int gfa2_get_stats(gfa2_handle h, gfa2_stats* s) {
     s->swing = st.swing;
     s->traction = st.traction;
     s->interactions = (int64_t)st.interactions;
+    s->interactions_total = (int64_t)st.interactions_total;
     s->cells = st.ncells;
     s->max_level = st.max_level;
     s->root_x = st.rootx;

@@ -39,6 +39,7 @@ struct DevState {
     float global_speed, speed_efficiency, jitter_tolerance, k_s_max;
     double swing, traction;
     unsigned long long interactions;
+    unsigned long long interactions_total;   // summed over all iterations since reset
     int ncells;
     int nrecords;
     int max_level;      // deepest cell level

@@ -255,6 +255,7 @@ __global__ __launch_bounds__(1024) void k_speed(const WalkPart* __restrict__ par
         st->swing = S;
         st->traction = T;
         st->interactions = ri[0];
+        st->interactions_total += ri[0];
         const float swg = (float)S, etra = (float)T;
         float gs = st->global_speed, se = st->speed_efficiency;
         const float jitter = st->jitter_tolerance, ksmax = st->k_s_max;

@@ -53,8 +53,11 @@ def edges_to_graph(s: np.ndarray, t: np.ndarray) -> Graph:
         return Graph(0, np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int64))
     # drop repeated undirected edges, keeping the first occurrence (src/RPGraph.cpp:52)
     lo, hi = np.minimum(s, t), np.maximum(s, t)
-    pair = np.stack([lo, hi], 1)
-    _, first = np.unique(pair, axis=0, return_index=True)
+    base = int(hi.max()) + 1 if len(hi) else 1
+    if int(lo.min()) >= 0 and base < (1 << 31):
+        _, first = np.unique(lo * base + hi, return_index=True)
+    else:
+        _, first = np.unique(np.stack([lo, hi], 1), axis=0, return_index=True)
     first.sort()
     s, t = s[first], t[first]
     # ids by first appearance, source before target (src/RPGraph.cpp:78-87, :89-94)
@@ -124,34 +127,26 @@ def rmat(scale: int, n_edges: int, a=0.57, b=0.19, c=0.19, seed: int = 1, batch:
     edges exist; returned in first-occurrence order with their drawn orientation."""
     rng = np.random.default_rng(seed)
     ab, abc = a + b, a + b + c
-    kept_s, kept_t = [], []
-    seen = np.zeros(0, np.int64)
-    have = 0
-    while have < n_edges:
-        u = rng.random((batch, scale))
+    all_s, all_t, all_c = [], [], []
+    distinct = 0
+    while distinct < n_edges:
         src = np.zeros(batch, np.int64)
         dst = np.zeros(batch, np.int64)
-        for lvl in range(scale):
-            x = u[:, lvl]
-            sbit = (x >= ab).astype(np.int64)                       # quadrants c, d
-            dbit = (((x >= a) & (x < ab)) | (x >= abc)).astype(np.int64)   # quadrants b, d
-            src = (src << 1) | sbit
-            dst = (dst << 1) | dbit
+        for _ in range(scale):
+            x = rng.random(batch)
+            src = (src << 1) | (x >= ab)                            # quadrants c, d
+            dst = (dst << 1) | (((x >= a) & (x < ab)) | (x >= abc))  # quadrants b, d
         ok = src != dst
         src, dst = src[ok], dst[ok]
-        code = np.minimum(src, dst) << scale | np.maximum(src, dst)
-        # first occurrence within the batch, then against everything kept so far
-        _, first = np.unique(code, return_index=True)
-        first.sort()
-        src, dst, code = src[first], dst[first], code[first]
-        new = ~np.isin(code, seen, assume_unique=False)
-        src, dst, code = src[new], dst[new], code[new]
-        take = min(len(src), n_edges - have)
-        kept_s.append(src[:take])
-        kept_t.append(dst[:take])
-        seen = np.union1d(seen, code[:take])
-        have += take
-    return np.concatenate(kept_s), np.concatenate(kept_t)
+        all_s.append(src)
+        all_t.append(dst)
+        all_c.append(np.minimum(src, dst) << scale | np.maximum(src, dst))
+        distinct = len(np.unique(np.concatenate(all_c)))
+    src, dst, code = np.concatenate(all_s), np.concatenate(all_t), np.concatenate(all_c)
+    _, first = np.unique(code, return_index=True)
+    first.sort()
+    first = first[:n_edges]
+    return src[first], dst[first]
 
 
 def sbm(n: int, blocks: int, alpha: float, seed: int):

@@ -57,6 +57,7 @@ typedef struct gfa2_stats {
     double swing;               /* S of the last iteration */
     double traction;            /* T of the last iteration */
     int64_t interactions;       /* body-node interactions of the last walk */
+    int64_t interactions_total; /* summed over all iterations since the last reset */
     int32_t cells;              /* quadtree cells of the last tree */
     int32_t max_level;          /* deepest cell level (root 0); max body depth = max_level + 1 */
     float root_x, root_y, radius;
