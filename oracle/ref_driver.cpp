@@ -7,6 +7,7 @@
 // oracle/_ref/libref.so; nothing of the reference is copied here. Tests use it to pin
 // the engine's loader and the CPU-baseline restatement.
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 #include "RPGraph.hpp"
@@ -39,6 +40,17 @@ void ref_coo(int* src, int* dst) {
     long long k = 0;
     for (uint32_t v = 0; v < g_graph->num_nodes(); ++v)
         for (uint32_t t : g_graph->neighbors_with_geq_id(v)) { src[k] = (int)v; dst[k] = (int)t; ++k; }
+}
+
+// The reference's initial layout draw: srandom(seed) (src/graph_viewer.cpp:49), then per
+// node x in [-w/2, w/2] and y in [-h/2, h/2] from RPGraph::get_random, as
+// GraphLayout::randomizePositions does (src/RPGraphLayout.cpp:45-52).
+void ref_random_positions(unsigned seed, int n, float w, float h, float* xy) {
+    srandom(seed);
+    for (int i = 0; i < n; ++i) {
+        xy[2 * i] = RPGraph::get_random(-w / 2.0, w / 2.0);
+        xy[2 * i + 1] = RPGraph::get_random(-h / 2.0, h / 2.0);
+    }
 }
 
 // The reference's CPU Barnes-Hut: insert n particles into a root (cx, cy, len) in order,

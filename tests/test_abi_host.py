@@ -1,0 +1,88 @@
+"""CPU checks of the C ABI library and the host-side logic (no device compute)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import gpugraphlayout_amd as g2
+from gpugraphlayout_amd import graphs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "gfa2.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gfa2_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_bound_symbols():
+    assert header_symbols() == sorted(g2.ABI_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(g2.LIB_PATH)
+    for name in header_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_library_host_only_calls():
+    lib = g2.lib()
+    assert lib.gfa2_strerror(0) == b"ok"
+    assert b"capacity" in lib.gfa2_strerror(-5)
+    p = g2.default_params()
+    assert p.use_barneshut == 1 and p.theta == 1.0 and p.group_size == 32
+    assert p.epssq == np.float32(0.05 * 0.05)
+    assert p.k_s_max == 10.0 and p.jitter_tolerance == 1.0 and p.global_speed == 1.0
+    # invalid arguments are rejected before any device work
+    h = ctypes.c_void_p()
+    bad = g2.default_params(group_size=48)
+    assert lib.gfa2_create(ctypes.byref(bad), 0, ctypes.byref(h)) != 0
+    assert lib.gfa2_step(None, 1) == -3
+
+
+def test_ba_generator_invariants():
+    s, t = graphs.barabasi_albert(1000, 5, 0)
+    assert len(s) == 5 + (1000 - 6) * 5
+    g = graphs.edges_to_graph(s, t)
+    assert g.n == 1000 and g.m == len(s)        # no duplicates, no loops
+    assert g.degrees().min() >= 1 and g.degrees().sum() == 2 * g.m
+    s2, t2 = graphs.barabasi_albert(1000, 5, 0)
+    assert np.array_equal(s, s2) and np.array_equal(t, t2)
+
+
+def test_rmat_generator_invariants():
+    s, t = graphs.rmat(12, 30000, seed=1)
+    assert len(s) == 30000 and np.all(s != t)
+    code = np.minimum(s, t) << 12 | np.maximum(s, t)
+    assert len(np.unique(code)) == 30000
+    assert s.max() < 4096 and t.max() < 4096
+    g = graphs.edges_to_graph(s, t)
+    assert g.m == 30000 and g.n <= 4096
+    # the skew: node 0 (all-zero bits, the most likely quadrant path) is the top hub
+    deg = np.bincount(np.concatenate([s, t]), minlength=4096)
+    assert deg.argmax() == 0
+
+
+def test_sbm_generator_invariants():
+    s, t, block = graphs.sbm(5000, 32, 2.0, 1000)
+    g = graphs.edges_to_graph(s, t)
+    assert g.m > 0 and len(block) >= 5000 - 64
+    same = block[s] == block[t]
+    assert same.mean() > 0.8       # p_in >> p_out
+
+
+def test_coo_to_csr_symmetric():
+    s, t = graphs.barabasi_albert(300, 3, 2)
+    g = graphs.edges_to_graph(s, t)
+    rowptr, col = g.csr()
+    assert rowptr[-1] == 2 * g.m
+    deg = np.diff(rowptr)
+    assert np.array_equal(deg, g.degrees())
+    pairs = set()
+    for v in range(g.n):
+        for u in col[rowptr[v]:rowptr[v + 1]]:
+            pairs.add((v, int(u)))
+    assert all((u, v) in pairs for v, u in pairs)
