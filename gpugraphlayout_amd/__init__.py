@@ -30,7 +30,7 @@ ABI_SYMBOLS = (
     "gfa2_set_positions", "gfa2_get_positions", "gfa2_reset", "gfa2_step", "gfa2_sync",
     "gfa2_get_forces", "gfa2_set_capture", "gfa2_get_tree", "gfa2_get_stats",
     "gfa2_set_timing", "gfa2_get_timing", "gfa2_set_graph_capture", "gfa2_destroy",
-    "gfa2_strerror", "gfa2_device_info",
+    "gfa2_strerror", "gfa2_device_info", "gfa2_set_params",
 )
 
 
@@ -98,6 +98,7 @@ def lib() -> ctypes.CDLL:
     L.gfa2_default_params.argtypes = [ctypes.POINTER(Params)]
     L.gfa2_default_params.restype = None
     L.gfa2_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(_P)]
+    L.gfa2_set_params.argtypes = [_P, ctypes.POINTER(Params)]
     L.gfa2_set_graph.argtypes = [_P, _I32, _P, _P, _P]
     L.gfa2_set_graph_coo.argtypes = [_P, _I32, _I64, _P, _P, _P]
     L.gfa2_set_positions.argtypes = [_P, _P]

@@ -208,6 +208,26 @@ int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out) {
     return GFA2_OK;
 }
 
+int gfa2_set_params(gfa2_handle h, const gfa2_params* prm) {
+    if (!h || !prm) return GFA2_ERR_INVALID_ARG;
+    if (!prm->use_barneshut) return GFA2_ERR_UNSUPPORTED;
+    if (prm->group_size != 32 && prm->group_size != 64) return GFA2_ERR_INVALID_ARG;
+    if (!(prm->theta > 0.0f) || !(prm->epssq >= 0.0f)) return GFA2_ERR_INVALID_ARG;
+    const int keep_factor = h->user.cell_factor;
+    h->user = *prm;
+    h->user.cell_factor = keep_factor;   // capacity is fixed by set_graph
+    h->p.strong_gravity = prm->strong_gravity;
+    h->p.k_g = prm->gravity;
+    h->p.k_r = prm->scale;
+    h->p.itolsq = 1.0f / (prm->theta * prm->theta);
+    h->p.epssq = prm->epssq;
+    h->p.group = prm->group_size;
+    h->graph_valid = false;
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    return GFA2_OK;
+}
+
 int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_t* col, const float* mass) {
     if (!h || !rowptr || n < 1) return n < 1 ? GFA2_ERR_INVALID_GRAPH : GFA2_ERR_INVALID_ARG;
     GFA2_TRY(hipSetDevice(h->device));

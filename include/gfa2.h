@@ -75,6 +75,12 @@ void gfa2_default_params(gfa2_params* p);
  * query, buffer sizing). device: HIP ordinal. */
 int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out);
 
+/* New parameter values for subsequent iterations: ForceAtlas2::setScale / setGravity
+ * (src/RPForceAtlas2.cpp:64-72) and the public strong_gravity flag
+ * (src/RPForceAtlas2.hpp:48). Speed state and graph are kept; theta, epssq and
+ * group_size may change too. */
+int gfa2_set_params(gfa2_handle h, const gfa2_params* params);
+
 /* Replaces host staging + H2D copies of masses and edges, src/RPGPUForceAtlas2.cu:51-84,
  * :152-161 and sendGraphToGPU :262-268. Symmetric CSR: rowptr[n+1] (int64), col[rowptr[n]]
  * holds every undirected edge in both rows. mass may be NULL: then deg+1

@@ -1,0 +1,38 @@
+/*
+ * gfa2host.h — C entry points of the host layer (libgfa2host.so): the edge-list loader and
+ * the layout writers behind the C++ classes of RPGraph.hpp / RPGraphLayout.hpp, for
+ * callers that are not C++ (bindings, tests). No device work.
+ *
+ *   gfa2host_load          UGraph(edgelist_path), src/RPGraph.cpp:35-56 (O(E) here)
+ *   gfa2host_graph_arrays  node_map_r, degree(), and the COO staging of
+ *                          src/RPGPUForceAtlas2.cu:73-84
+ *   gfa2host_write_layout  writeToCSV / writeToBin / writeToPNG, src/RPGraphLayout.cpp:170-250
+ */
+#ifndef GFA2HOST_H
+#define GFA2HOST_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gfa2host_graph* gfa2host_graph_t;
+
+/* Loads an edge list (or a .csrbin cache). Returns 0, or -1 if unreadable. */
+int gfa2host_load(const char* path, gfa2host_graph_t* out, int32_t* n, int64_t* m);
+/* Builds a graph from an in-memory edge list in file order (file ids). */
+int gfa2host_from_edges(int64_t count, const uint32_t* s, const uint32_t* t, gfa2host_graph_t* out, int32_t* n,
+                        int64_t* m);
+/* ids[n], deg[n], src[m], dst[m]; any may be NULL. */
+void gfa2host_graph_arrays(gfa2host_graph_t g, int64_t* ids, int32_t* deg, int32_t* src, int32_t* dst);
+/* Writes the binary cache (reloadable by gfa2host_load when named *.csrbin). */
+int gfa2host_save_binary(gfa2host_graph_t g, const char* path);
+/* fmt: "csv", "bin" or "png" (image_w x image_h). xy: interleaved positions. Refuses to
+ * overwrite an existing file (returns -1). */
+int gfa2host_write_layout(gfa2host_graph_t g, const float* xy, const char* path, const char* fmt, int32_t image_w,
+                          int32_t image_h);
+void gfa2host_free(gfa2host_graph_t g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
