@@ -31,6 +31,8 @@ ABI_SYMBOLS = (
     "gfa2_get_forces", "gfa2_set_capture", "gfa2_get_tree", "gfa2_get_stats",
     "gfa2_set_timing", "gfa2_get_timing", "gfa2_set_graph_capture", "gfa2_destroy",
     "gfa2_strerror", "gfa2_device_info", "gfa2_set_params",
+    "gfa2_stress_setup", "gfa2_stress_pairs", "gfa2_normalized_stress",
+    "gfa2_set_graph_batch", "gfa2_get_batch_stats",
 )
 
 
@@ -117,6 +119,11 @@ def lib() -> ctypes.CDLL:
     L.gfa2_strerror.argtypes = [ctypes.c_int]
     L.gfa2_strerror.restype = ctypes.c_char_p
     L.gfa2_device_info.argtypes = [ctypes.c_int, _P, _P, _P]
+    L.gfa2_stress_setup.argtypes = [_P, _I32, _I32, ctypes.c_uint64, _P]
+    L.gfa2_stress_pairs.argtypes = [_P, _I64, _P, _P, _P]
+    L.gfa2_normalized_stress.argtypes = [_P, _P, _P, _P]
+    L.gfa2_set_graph_batch.argtypes = [_P, _I32, _P, _P, _P, _P]
+    L.gfa2_get_batch_stats.argtypes = [_P, _I32, _P, _P, _P, _P, _P]
     for name in ABI_SYMBOLS:
         fn = getattr(L, name)
         if name not in ("gfa2_default_params", "gfa2_strerror"):
@@ -247,6 +254,30 @@ class Engine:
         _check(lib().gfa2_get_stats(self.h, ctypes.byref(s)), "gfa2_get_stats")
         return s.as_dict()
 
+    def stress_setup(self, nsources: int = 1024, targets_per_source: int = 977, seed: int = 0) -> int:
+        """Samples the pairs normalized stress is measured on (10^6 by default); returns the
+        number of reachable pairs."""
+        npairs = ctypes.c_int64(0)
+        _check(lib().gfa2_stress_setup(self.h, int(nsources), int(targets_per_source), int(seed),
+                                       ctypes.byref(npairs)), "gfa2_stress_setup")
+        self._stress_np = int(min(nsources, self.n)) * int(targets_per_source)
+        return npairs.value
+
+    def stress_pairs(self):
+        n = self._stress_np
+        a, b, d = (np.empty(n, np.int32) for _ in range(3))
+        _check(lib().gfa2_stress_pairs(self.h, n, _ptr(a), _ptr(b), _ptr(d)), "gfa2_stress_pairs")
+        return a, b, d
+
+    def normalized_stress(self, xy: Optional[np.ndarray] = None):
+        """(NS, alpha) of the device layout, or of the given positions."""
+        ns = ctypes.c_double(0)
+        alpha = ctypes.c_double(0)
+        p = None if xy is None else np.ascontiguousarray(xy, dtype=np.float32)
+        _check(lib().gfa2_normalized_stress(self.h, _ptr(p), ctypes.byref(ns), ctypes.byref(alpha)),
+               "gfa2_normalized_stress")
+        return ns.value, alpha.value
+
     def set_timing(self, on: bool = True):
         _check(lib().gfa2_set_timing(self.h, int(bool(on))), "gfa2_set_timing")
 
@@ -265,3 +296,44 @@ def device_info(device: int = 0) -> dict:
     _check(lib().gfa2_device_info(device, ctypes.byref(cus), ctypes.byref(wave), ctypes.byref(mem)),
            "gfa2_device_info")
     return {"cus": cus.value, "wave": wave.value, "mem_bytes": mem.value}
+
+
+class BatchEngine(Engine):
+    """Many independent graphs laid out together (gfa2_set_graph_batch). graphs: a list of
+    (n, src, dst) with local ids; positions are concatenated in list order."""
+
+    def __init__(self, graphs, masses=None, device: int = 0, **params):
+        self.params = default_params(**params)
+        h = ctypes.c_void_p()
+        _check(lib().gfa2_create(ctypes.byref(self.params), device, ctypes.byref(h)), "gfa2_create")
+        self.h = h
+        sizes = np.array([int(g[0]) for g in graphs], np.int64)
+        self.node_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+        self.n = int(self.node_off[-1])
+        self.ngraphs = len(graphs)
+        rows, cols = [np.zeros(1, np.int64)], []
+        base_e = 0
+        for k, (n, s, d) in enumerate(graphs):
+            rp, c = coo_to_csr(int(n), s, d)
+            rows.append(rp[1:] + base_e)
+            cols.append(c.astype(np.int64) + self.node_off[k])
+            base_e += int(rp[-1])
+        self._rowptr = np.ascontiguousarray(np.concatenate(rows), dtype=np.int64)
+        self._col = np.ascontiguousarray(np.concatenate(cols) if cols else np.zeros(0), dtype=np.int32)
+        m = None if masses is None else np.ascontiguousarray(np.concatenate(masses), dtype=np.float32)
+        _check(lib().gfa2_set_graph_batch(self.h, self.ngraphs, _ptr(self.node_off), _ptr(self._rowptr),
+                                          _ptr(self._col), _ptr(m)), "gfa2_set_graph_batch")
+
+    def batch_stats(self) -> dict:
+        G = self.ngraphs
+        speed = np.zeros(G, np.float32)
+        eff = np.zeros(G, np.float32)
+        swing = np.zeros(G, np.float64)
+        trac = np.zeros(G, np.float64)
+        inter = np.zeros(G, np.int64)
+        _check(lib().gfa2_get_batch_stats(self.h, G, _ptr(speed), _ptr(eff), _ptr(swing), _ptr(trac), _ptr(inter)),
+               "gfa2_get_batch_stats")
+        return {"speed": speed, "efficiency": eff, "swing": swing, "traction": trac, "interactions": inter}
+
+    def split(self, xy):
+        return [xy[self.node_off[k]:self.node_off[k + 1]] for k in range(self.ngraphs)]

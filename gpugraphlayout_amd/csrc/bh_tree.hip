@@ -98,6 +98,7 @@ __global__ __launch_bounds__(256) void k_bbox_partials(const float2* __restrict_
 // Root box + path keys. vals = identity.
 __global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, int n,
                                               const float4* __restrict__ part, DevState* st,
+                                              float4* __restrict__ seg_root,
                                               unsigned long long* __restrict__ keys,
                                               unsigned int* __restrict__ vals) {
     __shared__ float root[3];
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, in
         st->rootx = rootx;
         st->rooty = rooty;
         st->radius = radius;
+        seg_root[0] = make_float4(rootx, rooty, radius, 0.0f);
     }
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float2 p = pos[i];
@@ -129,10 +131,24 @@ __global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, in
     }
 }
 
-__device__ __forceinline__ int lcp_at(const unsigned long long* __restrict__ keys, int n, int i) {
+// A graph of a batch starts at sorted position i (always true for i == 0).
+__device__ __forceinline__ bool seg_start(const int* __restrict__ sg, int i) {
+    return i == 0 || (sg != nullptr && sg[i - 1] != sg[i]);
+}
+
+// Levels shared by sorted positions i and i+1; -1 at the end of a graph, except for a
+// graph of one body, whose root (level 0) opens at its only position.
+__device__ __forceinline__ int lcp_at(const unsigned long long* __restrict__ keys, const int* __restrict__ sg, int n,
+                                      int i) {
     if (i < 0) return -1;
-    if (i >= n - 1) return n == 1 ? 0 : -1;
+    const bool end = i >= n - 1 || (sg != nullptr && sg[i] != sg[i + 1]);
+    if (end) return seg_start(sg, i) ? 0 : -1;
     return lcp_levels(keys[i], keys[i + 1]);
+}
+
+// The level below which cells opened at i are shared with position i-1.
+__device__ __forceinline__ int lcp_prev(const signed char* __restrict__ lcp, const int* __restrict__ sg, int i) {
+    return seg_start(sg, i) ? -1 : (int)lcp[i - 1];
 }
 
 // Block-wide exclusive scan of one int per thread (256 threads). Returns the exclusive
@@ -161,20 +177,20 @@ __device__ int block_exclusive_scan(int v, int* total) {
 }
 
 // lcp per sorted position and the number of cells opened there; block sums.
-__global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* __restrict__ keys, int n,
+__global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* __restrict__ keys,
+                                                      const int* __restrict__ sg, int n,
                                                       signed char* __restrict__ lcp,
                                                       int* __restrict__ blocksum) {
     const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
-    int prev = lcp_at(keys, n, i0 - 1);
     int sum = 0;
 #pragma unroll
     for (int k = 0; k < kTreeItems; ++k) {
         int i = i0 + k;
         if (i < n) {
-            int l = lcp_at(keys, n, i);
+            const int prev = seg_start(sg, i) ? -1 : lcp_at(keys, sg, n, i - 1);
+            int l = lcp_at(keys, sg, n, i);
             lcp[i] = (signed char)l;
             sum += max(0, l - prev);
-            prev = l;
         }
     }
     int total;
@@ -185,7 +201,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
 // Single block: exclusive scan of block sums (in place), cell total, capacity check,
 // per-iteration counters reset.
 __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum, int nb, int n,
-                                                      int cap_records, DevState* st) {
+                                                      int cap_records, int* __restrict__ rec_end, DevState* st) {
     __shared__ int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int per = (nb + 1023) / 1024;
@@ -217,6 +233,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum
     if (tid == 0) {
         st->ncells = all;
         st->nrecords = n + all;
+        *rec_end = n + all;
         if (n + all > cap_records) st->err |= kErrTreeOverflow;
         st->max_level = 0;
     }
@@ -227,7 +244,8 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum
 }
 
 // S (exclusive prefix of opened cells), body records and brec.
-__global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __restrict__ lcp, int n,
+__global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __restrict__ lcp,
+                                                         const int* __restrict__ sg, int n,
                                                          const int* __restrict__ blockoff,
                                                          int* __restrict__ S,
                                                          const unsigned int* __restrict__ vals,
@@ -237,14 +255,11 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
                                                          const DevState* __restrict__ st) {
     const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
     int s[kTreeItems];
-    int prev = (i0 - 1 >= 0 && i0 - 1 < n) ? (int)lcp[i0 - 1] : -1;
     int sum = 0;
 #pragma unroll
     for (int k = 0; k < kTreeItems; ++k) {
         int i = i0 + k;
-        int l = i < n ? (int)lcp[i] : prev;
-        s[k] = max(0, l - prev);
-        prev = l;
+        s[k] = i < n ? max(0, (int)lcp[i] - lcp_prev(lcp, sg, i)) : 0;
         sum += s[k];
     }
     int total;
@@ -270,10 +285,12 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
 
 // Cell records (skip, level, count) and the cell list for level bucketing.
 __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long* __restrict__ keys,
-                                                        const signed char* __restrict__ lcp, int n,
+                                                        const signed char* __restrict__ lcp,
+                                                        const int* __restrict__ sg, int n,
                                                         const int* __restrict__ S, float4* __restrict__ rec,
                                                         int* __restrict__ rcount, int* __restrict__ cell_rec,
-                                                        unsigned char* __restrict__ cell_lvl, DevState* st) {
+                                                        unsigned char* __restrict__ cell_lvl, int* __restrict__ rec_begin,
+                                                        DevState* st) {
     __shared__ int hist[kMaxLevels + 1];
     __shared__ int lmax;
     if (threadIdx.x <= kMaxLevels) hist[threadIdx.x] = 0;
@@ -282,17 +299,19 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
     const bool ok = st->err == 0;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (ok && i < n) {
-        const int lprev = i > 0 ? (int)lcp[i - 1] : -1;
+        const int lprev = lcp_prev(lcp, sg, i);
         const int lcur = (int)lcp[i];
         if (lcur > lprev) {
             const unsigned long long ki = keys[i];
+            const int gi = sg ? sg[i] : 0;
             const int Si = S[i];
             int lo = i;     // known member of the current cell's extent
             for (int L = lcur; L > lprev; --L) {
-                // extent end: largest j with lcp(k_i, k_j) >= L (keys sorted => contiguous)
+                // extent end: largest j of the same graph with lcp(k_i, k_j) >= L
+                // (keys sorted within a graph => contiguous)
                 int step = 1;
                 int hi = lo + 1;
-                while (hi < n && lcp_levels(ki, keys[hi]) >= L) {
+                while (hi < n && (!sg || sg[hi] == gi) && lcp_levels(ki, keys[hi]) >= L) {
                     lo = hi;
                     step <<= 1;
                     hi = lo + step;
@@ -300,7 +319,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
                 if (hi > n) hi = n;
                 while (hi - lo > 1) {
                     int mid = lo + ((hi - lo) >> 1);
-                    if (lcp_levels(ki, keys[mid]) >= L) lo = mid;
+                    if ((!sg || sg[mid] == gi) && lcp_levels(ki, keys[mid]) >= L) lo = mid;
                     else hi = mid;
                 }
                 const int b = lo;
@@ -312,6 +331,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
                 rcount[rc] = b - i + 1;
                 cell_rec[ord] = rc;
                 cell_lvl[ord] = (unsigned char)L;
+                if (L == 0 && rec_begin) rec_begin[gi] = rc;
                 atomicAdd(&hist[L], 1);
             }
             atomicMax(&lmax, lcur);
@@ -398,18 +418,20 @@ void launch_bbox_partials(const Bufs& b, hipStream_t s) {
 void launch_keys(const Bufs& b, hipStream_t s) {
     int blocks = (b.n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
-    k_keys<<<blocks, 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st, b.sort.keys[0], b.sort.vals[0]);
+    k_keys<<<blocks, 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st, b.seg_root, b.sort.keys[0], b.sort.vals[0]);
 }
 
 void launch_tree(Bufs& b, hipStream_t s) {
     const int n = b.n;
     const int tiles = (n + kTreeTile - 1) / kTreeTile;
-    k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], n, b.lcp, b.blocksum);
-    k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.st);
-    k_prefix<<<tiles, kTreeThreads, 0, s>>>(b.lcp, n, b.blocksum, b.S, b.sort.vals[0], b.pos, b.mass, b.rec,
+    const int* sg = b.nseg > 1 ? b.sg : nullptr;
+    k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum);
+    k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st);
+    k_prefix<<<tiles, kTreeThreads, 0, s>>>(b.lcp, sg, n, b.blocksum, b.S, b.sort.vals[0], b.pos, b.mass, b.rec,
                                             b.brec, b.st);
-    k_nodes<<<(n + kTreeThreads - 1) / kTreeThreads, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, n, b.S, b.rec,
-                                                                           b.rcount, b.cell_rec, b.cell_lvl, b.st);
+    k_nodes<<<(n + kTreeThreads - 1) / kTreeThreads, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, sg, n, b.S, b.rec,
+                                                                           b.rcount, b.cell_rec, b.cell_lvl,
+                                                                           b.rec_begin, b.st);
     const int max_cells = b.cap_records - n;
     k_levels<<<(max_cells + 255) / 256, 256, 0, s>>>(b.cell_rec, b.cell_lvl, b.level_list, b.st);
     int sum_blocks = (max_cells / 4 + 255) / 256;

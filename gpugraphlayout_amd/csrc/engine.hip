@@ -42,6 +42,16 @@ struct gfa2_engine {
     bool graph_valid = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // layout-quality pairs (gfa2_stress_setup)
+    int stress_K = 0, stress_T = 0;
+    int* stress_src = nullptr;
+    int* stress_tgt = nullptr;
+    int* stress_dist = nullptr;
+    double* stress_part = nullptr;
+    float2* stress_pos = nullptr;
+    // graphs of a batch (one for a single graph)
+    std::vector<int> node_off_host;
+    int groups_cap = 0;
 };
 
 namespace {
@@ -78,7 +88,25 @@ void free_all(gfa2_engine* h) {
     h->b = Bufs{};
     h->has_graph = false;
     h->has_pos = false;
+    h->stress_K = h->stress_T = 0;
+    h->stress_src = h->stress_tgt = h->stress_dist = nullptr;
+    h->stress_part = nullptr;
+    h->stress_pos = nullptr;
 }
+
+// splitmix64: the engine's own seeded generator for pair sampling.
+struct SplitMix {
+    uint64_t x;
+    uint64_t next() {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    uint64_t below(uint64_t r) { return next() % r; }
+};
+
+constexpr int kStressBlocks = 512;
 
 void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
     hipStream_t s = h->stream;
@@ -86,9 +114,15 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
     launch_attraction(b, h->p, s);
     if (ev) hipEventRecord(ev[1], s);
-    launch_keys(b, s);
+    if (b.nseg > 1) {
+        launch_seg_bbox(b, s);
+        launch_seg_keys(b, s);
+    } else {
+        launch_keys(b, s);
+    }
     if (ev) hipEventRecord(ev[2], s);
     radix_sort_pairs(b.sort, b.n, 0, 64, s);
+    if (b.nseg > 1) launch_seg_graph_sort(b, s);
     if (ev) hipEventRecord(ev[3], s);
     launch_tree(b, s);
     if (ev) hipEventRecord(ev[4], s);
@@ -124,6 +158,15 @@ int reset_state(gfa2_engine* h) {
     st.jitter_tolerance = h->user.jitter_tolerance;
     st.k_s_max = h->user.k_s_max;
     GFA2_TRY(hipMemcpyAsync(h->b.st, &st, sizeof(DevState), hipMemcpyHostToDevice, h->stream));
+    std::vector<SegState> seg((size_t)h->b.nseg);
+    for (SegState& q : seg) {
+        q = SegState{};
+        q.global_speed = h->user.global_speed;
+        q.speed_efficiency = h->user.speed_efficiency;
+        q.jitter_tolerance = h->user.jitter_tolerance;
+        q.k_s_max = h->user.k_s_max;
+    }
+    GFA2_TRY(hipMemcpyAsync(h->b.seg_st, seg.data(), sizeof(SegState) * seg.size(), hipMemcpyHostToDevice, h->stream));
     GFA2_TRY(hipMemsetAsync(h->b.fprev, 0, sizeof(float2) * h->b.n, h->stream));
     GFA2_TRY(hipMemsetAsync(h->b.f, 0, sizeof(float2) * h->b.n, h->stream));
     GFA2_TRY(hipMemsetAsync(h->b.fga, 0, sizeof(float2) * h->b.n, h->stream));
@@ -131,6 +174,8 @@ int reset_state(gfa2_engine* h) {
     GFA2_TRY(hipStreamSynchronize(h->stream));
     return GFA2_OK;
 }
+
+int build_groups(gfa2_engine* h);
 
 // Copies DevState back (waits for the stream).
 int read_state(gfa2_engine* h, DevState* st) {
@@ -221,31 +266,69 @@ int gfa2_set_params(gfa2_handle h, const gfa2_params* prm) {
     h->p.k_r = prm->scale;
     h->p.itolsq = 1.0f / (prm->theta * prm->theta);
     h->p.epssq = prm->epssq;
+    const bool regroup = h->p.group != prm->group_size;
     h->p.group = prm->group_size;
+    if (regroup && h->has_graph) {
+        const int rc = build_groups(h);
+        if (rc != GFA2_OK) return rc;
+    }
     h->graph_valid = false;
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
     return GFA2_OK;
 }
 
-int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_t* col, const float* mass) {
-    if (!h || !rowptr || n < 1) return n < 1 ? GFA2_ERR_INVALID_GRAPH : GFA2_ERR_INVALID_ARG;
+}  // extern "C"
+
+namespace {
+
+// Walk groups: per graph, runs of G sorted positions starting at the graph's first.
+int build_groups(gfa2_engine* h) {
+    Bufs& b = h->b;
+    const int G = h->p.group;
+    std::vector<int2> groups;
+    std::vector<int> first((size_t)b.nseg + 1);
+    for (int g = 0; g < b.nseg; ++g) {
+        first[g] = (int)groups.size();
+        for (int k = h->node_off_host[g]; k < h->node_off_host[g + 1]; k += G) groups.push_back(make_int2(k, g));
+    }
+    first[b.nseg] = (int)groups.size();
+    if ((int)groups.size() > h->groups_cap) return GFA2_ERR_INVALID_ARG;
+    b.ngroups = (int)groups.size();
+    GFA2_TRY(hipMemcpyAsync(b.groups, groups.data(), sizeof(int2) * groups.size(), hipMemcpyHostToDevice, h->stream));
+    GFA2_TRY(hipMemcpyAsync(b.grp_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, h->stream));
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    return GFA2_OK;
+}
+
+// Validates a symmetric CSR whose edges stay inside their graph, allocates every buffer and
+// uploads the graph. node_off: nseg+1 node offsets (nseg = 1 for a single graph).
+int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32_t* col, const float* mass,
+                   int nseg, const int32_t* node_off) {
+    if (n < 1 || nseg < 1 || nseg > n) return GFA2_ERR_INVALID_GRAPH;
     GFA2_TRY(hipSetDevice(h->device));
     if (rowptr[0] != 0) return GFA2_ERR_INVALID_GRAPH;
+    if (node_off[0] != 0 || node_off[nseg] != n) return GFA2_ERR_INVALID_GRAPH;
+    for (int g = 0; g < nseg; ++g)
+        if (node_off[g + 1] <= node_off[g]) return GFA2_ERR_INVALID_GRAPH;
     const int64_t nnz = rowptr[n];
     if (nnz < 0 || nnz >= (int64_t)1 << 31 || (nnz > 0 && !col)) return GFA2_ERR_INVALID_GRAPH;
     std::vector<int> rp32((size_t)n + 1);
     std::vector<int> lists;
     lists.reserve(n);
-    std::vector<int> medium, heavy;
+    std::vector<int> medium, heavy, seg_of;
     std::vector<float> m((size_t)n);
+    if (nseg > 1) seg_of.resize(n);
+    int g = 0;
     for (int v = 0; v < n; ++v) {
-        const int64_t a = rowptr[v], b = rowptr[v + 1];
-        if (b < a || b > nnz) return GFA2_ERR_INVALID_GRAPH;
-        for (int64_t e = a; e < b; ++e)
-            if (col[e] < 0 || col[e] >= n || col[e] == v) return GFA2_ERR_INVALID_GRAPH;
+        while (v >= node_off[g + 1]) ++g;
+        if (nseg > 1) seg_of[v] = g;
+        const int64_t a = rowptr[v], e1 = rowptr[v + 1];
+        if (e1 < a || e1 > nnz) return GFA2_ERR_INVALID_GRAPH;
+        for (int64_t e = a; e < e1; ++e)
+            if (col[e] < node_off[g] || col[e] >= node_off[g + 1] || col[e] == v) return GFA2_ERR_INVALID_GRAPH;
         rp32[v] = (int)a;
-        const int64_t deg = b - a;
+        const int64_t deg = e1 - a;
         if (deg <= 32) lists.push_back(v);
         else if (deg <= 2048) medium.push_back(v);
         else heavy.push_back(v);
@@ -265,6 +348,13 @@ int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_
     b.n = n;
     b.nnz = nnz;
     b.cap_records = (int)cap;
+    b.nseg = nseg;
+    b.seg_bits = 0;
+    while ((1 << b.seg_bits) < nseg) ++b.seg_bits;
+    h->node_off_host.assign(node_off, node_off + nseg + 1);
+    int groups_cap = 0;
+    for (int q = 0; q < nseg; ++q) groups_cap += (node_off[q + 1] - node_off[q] + 31) / 32;
+    h->groups_cap = groups_cap;
     const int sort_tiles = (n + kSortTile - 1) / kSortTile;
     const int tree_tiles = (n + kTreeTile - 1) / kTreeTile;
     const size_t maxcells = (size_t)(cap - n);
@@ -295,9 +385,22 @@ int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_
     GFA2_ALLOC(b.cell_lvl, maxcells);
     GFA2_ALLOC(b.level_list, maxcells);
     GFA2_ALLOC(b.bbox_part, kPartBlocks);
-    b.nwalk_part = walk_blocks(n, h->p.group);
-    GFA2_ALLOC(b.walk_part, b.nwalk_part);
     GFA2_ALLOC(b.st, 1);
+    GFA2_ALLOC(b.node_off, (size_t)nseg + 1);
+    GFA2_ALLOC(b.seg_root, nseg);
+    GFA2_ALLOC(b.rec_begin, (size_t)nseg + 1);
+    GFA2_ALLOC(b.groups, groups_cap);
+    GFA2_ALLOC(b.grp_first, (size_t)nseg + 1);
+    GFA2_ALLOC(b.seg_st, nseg);
+    GFA2_ALLOC(b.grp_part, groups_cap);
+    if (nseg > 1) {
+        int* so = nullptr;
+        GFA2_ALLOC(so, n);
+        b.seg_of = so;
+        GFA2_ALLOC(b.sg, n);
+        GFA2_ALLOC(b.pkey, n);
+        GFA2_TRY(hipMemcpyAsync(so, seg_of.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+    }
     if (h->capture_rep) GFA2_ALLOC(b.rep, n);
 #undef GFA2_ALLOC
     b.nlight = nlight;
@@ -308,11 +411,50 @@ int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_
         GFA2_TRY(hipMemcpyAsync(b.col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, h->stream));
     GFA2_TRY(hipMemcpyAsync(b.mass, m.data(), sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
     GFA2_TRY(hipMemcpyAsync(b.node_lists, lists.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+    GFA2_TRY(hipMemcpyAsync(b.node_off, node_off, sizeof(int) * (nseg + 1), hipMemcpyHostToDevice, h->stream));
     GFA2_TRY(hipMemsetAsync(b.pos, 0, sizeof(float2) * n, h->stream));
+    GFA2_TRY(hipMemsetAsync(b.rec_begin, 0, sizeof(int) * (nseg + 1), h->stream));
     h->edges = nnz / 2;
     h->has_graph = true;
     h->has_pos = false;
+    st = build_groups(h);
+    if (st != GFA2_OK) return st;
     return reset_state(h);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_t* col, const float* mass) {
+    if (!h || !rowptr) return GFA2_ERR_INVALID_ARG;
+    const int32_t off[2] = {0, n};
+    return set_graph_impl(h, n, rowptr, col, mass, 1, off);
+}
+
+int gfa2_set_graph_batch(gfa2_handle h, int32_t ngraphs, const int32_t* node_off, const int64_t* rowptr,
+                         const int32_t* col, const float* mass) {
+    if (!h || !rowptr || !node_off || ngraphs < 1) return GFA2_ERR_INVALID_ARG;
+    return set_graph_impl(h, node_off[ngraphs], rowptr, col, mass, ngraphs, node_off);
+}
+
+int gfa2_get_batch_stats(gfa2_handle h, int32_t cap, float* speed, float* efficiency, double* swing,
+                         double* traction, int64_t* interactions) {
+    if (!h || cap < 0) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    const int ns = h->b.nseg < cap ? h->b.nseg : cap;
+    std::vector<SegState> seg((size_t)h->b.nseg);
+    GFA2_TRY(hipMemcpyAsync(seg.data(), h->b.seg_st, sizeof(SegState) * h->b.nseg, hipMemcpyDeviceToHost, h->stream));
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    for (int g = 0; g < ns; ++g) {
+        if (speed) speed[g] = seg[g].global_speed;
+        if (efficiency) efficiency[g] = seg[g].speed_efficiency;
+        if (swing) swing[g] = seg[g].swing;
+        if (traction) traction[g] = seg[g].traction;
+        if (interactions) interactions[g] = (int64_t)seg[g].interactions;
+    }
+    return GFA2_OK;
 }
 
 int gfa2_set_graph_coo(gfa2_handle h, int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
@@ -534,6 +676,104 @@ int gfa2_set_graph_capture(gfa2_handle h, int32_t on) {
     h->graph_valid = false;
     if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+    return GFA2_OK;
+}
+
+int gfa2_stress_setup(gfa2_handle h, int32_t nsources, int32_t targets_per_source, uint64_t seed, int64_t* npairs) {
+    if (!h || nsources < 1 || targets_per_source < 1) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    const int n = h->b.n;
+    if (n < 2) return GFA2_ERR_INVALID_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    const int K = nsources < n ? nsources : n;
+    const int T = targets_per_source;
+    if ((long long)K * T >= (1ll << 31)) return GFA2_ERR_INVALID_ARG;
+    // sources: the first K entries of a seeded Fisher-Yates shuffle; targets: uniform over
+    // the other n-1 nodes, with replacement
+    SplitMix rng{seed};
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = 0; i < K; ++i) {
+        const int j = i + (int)rng.below((uint64_t)(n - i));
+        std::swap(perm[i], perm[j]);
+    }
+    std::vector<int> src(perm.begin(), perm.begin() + K), tgt((size_t)K * T);
+    for (int k = 0; k < K; ++k)
+        for (int j = 0; j < T; ++j) {
+            int u = (int)rng.below((uint64_t)(n - 1));
+            tgt[(size_t)k * T + j] = u >= src[k] ? u + 1 : u;
+        }
+    int st = GFA2_OK;
+    if (!h->stress_part) {
+        if ((st = dalloc(h, &h->stress_part, 3 * kStressBlocks)) != GFA2_OK) return st;
+        if ((st = dalloc(h, &h->stress_pos, n)) != GFA2_OK) return st;
+    }
+    if (K * T > h->stress_K * h->stress_T) {
+        if ((st = dalloc(h, &h->stress_tgt, (size_t)K * T)) != GFA2_OK) return st;
+        if ((st = dalloc(h, &h->stress_dist, (size_t)K * T)) != GFA2_OK) return st;
+    }
+    if (K > h->stress_K && (st = dalloc(h, &h->stress_src, K)) != GFA2_OK) return st;
+    h->stress_K = K;
+    h->stress_T = T;
+    GFA2_TRY(hipMemcpyAsync(h->stress_src, src.data(), sizeof(int) * K, hipMemcpyHostToDevice, h->stream));
+    GFA2_TRY(hipMemcpyAsync(h->stress_tgt, tgt.data(), sizeof(int) * tgt.size(), hipMemcpyHostToDevice, h->stream));
+    // BFS scratch, freed right after
+    unsigned long long *seen = nullptr, *front = nullptr, *next = nullptr;
+    int* flag = nullptr;
+    GFA2_TRY(hipMalloc(&seen, sizeof(unsigned long long) * n));
+    GFA2_TRY(hipMalloc(&front, sizeof(unsigned long long) * n));
+    GFA2_TRY(hipMalloc(&next, sizeof(unsigned long long) * n));
+    GFA2_TRY(hipMalloc(&flag, sizeof(int)));
+    int rc = stress_hop_distances(h->b, h->stress_src, K, h->stress_tgt, T, h->stress_dist, seen, front, next, flag,
+                                  h->stream);
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(seen);
+    (void)hipFree(front);
+    (void)hipFree(next);
+    (void)hipFree(flag);
+    if (rc != 0) return GFA2_ERR_HIP;
+    if (npairs) {
+        std::vector<int> d((size_t)K * T);
+        GFA2_TRY(hipMemcpy(d.data(), h->stress_dist, sizeof(int) * d.size(), hipMemcpyDeviceToHost));
+        int64_t c = 0;
+        for (int v : d) c += v > 0;
+        *npairs = c;
+    }
+    return GFA2_OK;
+}
+
+int gfa2_stress_pairs(gfa2_handle h, int64_t cap, int32_t* a, int32_t* b, int32_t* d) {
+    if (!h || cap < 0) return GFA2_ERR_INVALID_ARG;
+    if (!h->stress_K) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    const int64_t np = (int64_t)h->stress_K * h->stress_T;
+    const int64_t c = np < cap ? np : cap;
+    std::vector<int> src(h->stress_K);
+    GFA2_TRY(hipMemcpy(src.data(), h->stress_src, sizeof(int) * h->stress_K, hipMemcpyDeviceToHost));
+    if (b) GFA2_TRY(hipMemcpy(b, h->stress_tgt, sizeof(int) * c, hipMemcpyDeviceToHost));
+    if (d) GFA2_TRY(hipMemcpy(d, h->stress_dist, sizeof(int) * c, hipMemcpyDeviceToHost));
+    if (a)
+        for (int64_t p = 0; p < c; ++p) a[p] = src[p / h->stress_T];
+    return GFA2_OK;
+}
+
+int gfa2_normalized_stress(gfa2_handle h, const float* xy, double* ns, double* alpha) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    if (!h->stress_K) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    const float2* pos = h->b.pos;
+    if (xy) {
+        GFA2_TRY(hipMemcpyAsync(h->stress_pos, xy, sizeof(float2) * h->b.n, hipMemcpyHostToDevice, h->stream));
+        pos = h->stress_pos;
+    }
+    double s[3];
+    if (stress_reduce(pos, h->stress_src, h->stress_tgt, h->stress_dist, h->stress_T,
+                      (long long)h->stress_K * h->stress_T, h->stress_part, kStressBlocks, s, h->stream) != 0)
+        return GFA2_ERR_HIP;
+    const double P = s[2];
+    if (ns) *ns = (P > 0 && s[1] > 0) ? 1.0 - s[0] * s[0] / (P * s[1]) : 0.0;
+    if (alpha) *alpha = s[1] > 0 ? s[0] / s[1] : 0.0;
     return GFA2_OK;
 }
 

@@ -101,7 +101,36 @@ struct Bufs {
     WalkPart* walk_part;
     int nwalk_part;
     DevState* st;
+    // Batched layout of many graphs (nseg > 1). Graph g owns nodes [node_off[g],
+    // node_off[g+1]) of the concatenated input and, after the sort, the same range of sorted
+    // positions; its records are [rec_begin[g], rec_begin[g+1]).
+    int nseg = 1;
+    int seg_bits = 0;           // bits of a graph id
+    const int* seg_of = nullptr;      // node -> graph
+    int* node_off = nullptr;          // nseg + 1
+    int* sg = nullptr;                // sorted position -> graph
+    unsigned long long* pkey = nullptr;   // path key per node (regathered after the graph sort)
+    float4* seg_root = nullptr;       // {rootx, rooty, radius, 0} per graph
+    int* rec_begin = nullptr;         // nseg + 1
+    int2* groups = nullptr;           // walk groups {first sorted position, graph}, per graph in order
+    int ngroups = 0;
+    int* grp_first = nullptr;         // graph -> first group, nseg + 1
+    struct SegState* seg_st = nullptr;
+    WalkPart* grp_part = nullptr;     // per group
 };
+
+// Per-graph FA2 state of a batch.
+struct SegState {
+    float global_speed, speed_efficiency, jitter_tolerance, k_s_max;
+    double swing, traction;
+    unsigned long long interactions;
+    int iteration, pad;
+};
+
+// Batched-mode launch groups (batch.hip).
+void launch_seg_bbox(const Bufs& b, hipStream_t s);
+void launch_seg_keys(const Bufs& b, hipStream_t s);
+void launch_seg_graph_sort(Bufs& b, hipStream_t s);     // after the path sort
 
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
@@ -123,8 +152,16 @@ void launch_walk(const Bufs& b, const Params& p, hipStream_t s);
 void launch_speed(const Bufs& b, hipStream_t s);
 void launch_displacement(const Bufs& b, hipStream_t s);
 
+// Layout quality (stress.hip): hop distances of K*T sampled pairs by multi-source BFS,
+// and the normalized-stress sums {sum r, sum r^2, pairs} for positions pos.
+int stress_hop_distances(const Bufs& b, const int* d_src, int K, const int* d_tgt, int T, int* d_dist,
+                         unsigned long long* seen, unsigned long long* front, unsigned long long* next, int* d_flag,
+                         hipStream_t s);
+int stress_reduce(const float2* pos, const int* d_src, const int* d_tgt, const int* d_dist, int T, long long np,
+                  double* d_part, int nblocks, double out[3], hipStream_t s);
+
 // Walk grid sizes (partials count depends on them).
-int walk_blocks(int n, int group);
+int walk_blocks(int ngroups, int group);
 int displacement_blocks(int n);
 
 }  // namespace gfa2

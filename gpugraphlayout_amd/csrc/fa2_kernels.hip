@@ -103,48 +103,50 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
 // ---------------------------------------------------------------------------------------
 // Barnes-Hut repulsion walk. Reference: ForceCalculationKernel, src/RPBHKernels.cu:566-693.
 //
-// Groups of G consecutive bodies of the sorted (in-order) permutation walk the tree
-// together. G = 32 reproduces the reference's 32-lane warp vote on wave64 by giving each
-// half-wave its own walk; G = 64 votes over the whole wave. The tree is the pre-order
-// record array, so the walk needs no stack: from record r, a body is always taken
-// directly (r+1); a cell is taken as a whole when every active lane of the group has
-// dx^2+dy^2+epssq >= dq[level-1] (r += skip), and otherwise opened (r+1). That visits
+// Groups of G consecutive bodies of a graph's sorted (in-order) permutation walk that
+// graph's tree together; groups start at the graph's first sorted position, as the
+// reference's warps start at position 0 (a batch of graphs is laid out exactly as each
+// graph alone). G = 32 reproduces the reference's 32-lane warp vote on wave64 by giving
+// each half-wave its own group; G = 64 votes over the whole wave. The tree is the
+// pre-order record array, so the walk needs no stack: from record r, a body is always
+// taken directly (r+1); a cell is taken as a whole when every active lane of the group
+// has dx^2+dy^2+epssq >= dq[level-1] (r += skip), and otherwise opened (r+1). That visits
 // children in quadrant order and accumulates in the reference's DFS order.
 // dq[d] = (2*radius)^2*itolsq*0.25^d + epssq (src/RPBHKernels.cu:576-590) is formed with
 // ldexp, which is exact for the power-of-4 scaling.
 // The term k_r*m_i*m_n*dx/(d^2+eps^2) uses a hardware reciprocal (the reference divides);
 // the difference is an ulp or two per term.
-// The epilogue adds gravity + attraction, writes the total force, and forms this
-// group's swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161) so the speed
-// step reads only per-block partials.
+// The epilogue adds gravity + attraction, writes the total force, and forms the group's
+// swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
 // ---------------------------------------------------------------------------------------
 template <int G>
 __global__ __launch_bounds__(256) void k_walk(const float4* __restrict__ rec, const int* __restrict__ brec,
-                                              const unsigned int* __restrict__ perm, int n,
+                                              const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
+                                              int ngroups, const int* __restrict__ node_off,
+                                              const int* __restrict__ rec_begin, const float4* __restrict__ seg_root,
                                               const float2* __restrict__ fga, const float2* __restrict__ fprev,
                                               float2* __restrict__ f, float2* __restrict__ rep,
                                               WalkPart* __restrict__ part, float k_r, float itolsq, float epssq,
                                               const DevState* __restrict__ st) {
-    __shared__ float red_s[4], red_t[4];
-    __shared__ unsigned int red_i[4];
     if (st->err != 0) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
-    // XCD-aware block order: consecutive sorted ranges stay on one XCD's L2.
+    // XCD-aware block order: consecutive groups stay on one XCD's L2.
     const int nb = gridDim.x;
     const int xcd = blockIdx.x & 7;
     const int per = (nb + 7) >> 3;
-    const int full = nb & 7;   // number of XCDs that get 'per' blocks when nb % 8 != 0
+    const int full = nb & 7;
     int lb;
     if (full == 0) lb = xcd * per + (blockIdx.x >> 3);
     else lb = (xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1)) + (blockIdx.x >> 3);
-    const int k = lb * 256 + tid;
-    const bool active = k < n;
-
-    const int nrec = st->nrecords;
-    const float two_r = st->radius * 2;
-    const float base = two_r * two_r * itolsq;
+    const int gidx = (lb * 256 + tid) / G;
+    const int gl = (lb * 256 + tid) % G;          // lane within the group
+    const bool group_ok = gidx < ngroups;
+    int2 grp = make_int2(0, 0);
+    if (group_ok) grp = groups[gidx];
+    const int g = grp.y;
+    const int k = grp.x + gl;
+    const bool active = group_ok && k < node_off[g + 1];
 
     float px = 0.0f, py = 0.0f, mi = 0.0f, kmi = 0.0f;
     unsigned int id = 0;
@@ -160,10 +162,12 @@ __global__ __launch_bounds__(256) void k_walk(const float4* __restrict__ rec, co
     unsigned int inter = 0;
     // Every lane of a group walks, active or not, so the group's record index stays uniform.
     const int gbase = (G == 32) ? (lane & 32) : 0;
-    const bool group_live = __ballot(active) != 0ull;
-    int r = 1;
-    if (group_live) {
-        while (r < nrec) {
+    if (group_ok) {
+        const float two_r = seg_root[g].z * 2;
+        const float base = two_r * two_r * itolsq;
+        int r = rec_begin[g] + 1;
+        const int end = rec_begin[g + 1];
+        while (r < end) {
             const float4 c = rec[r];
             const unsigned int info = __float_as_uint(c.w);
             const float dx = px - c.x;
@@ -198,42 +202,43 @@ __global__ __launch_bounds__(256) void k_walk(const float4* __restrict__ rec, co
         dy = F.y + P.y;
         tr = mi * (sqrtf(dx * dx + dy * dy) * 0.5f);
     }
+    // Reduce within the group (xor offsets below G stay inside it).
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = G / 2; o > 0; o >>= 1) {
         sw += __shfl_xor(sw, o, 64);
         tr += __shfl_xor(tr, o, 64);
         inter += __shfl_xor(inter, o, 64);
     }
-    if (lane == 0) {
-        red_s[wave] = sw;
-        red_t[wave] = tr;
-        red_i[wave] = inter;
-    }
-    __syncthreads();
-    if (tid == 0) {
+    if (group_ok && gl == 0) {
         WalkPart wp;
-        wp.swing = (red_s[0] + red_s[1]) + (red_s[2] + red_s[3]);
-        wp.traction = (red_t[0] + red_t[1]) + (red_t[2] + red_t[3]);
-        wp.interactions = red_i[0] + red_i[1] + red_i[2] + red_i[3];
+        wp.swing = sw;
+        wp.traction = tr;
+        wp.interactions = inter;
         wp.pad = 0;
-        part[lb] = wp;
+        part[gidx] = wp;
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// Speed update. Reference: SpeedKernel's last block, src/RPFA2Kernels.cu:167-219 (Gephi's
-// adaptive speed). The block partials are summed in double in a fixed order; the update
-// itself is the reference's float arithmetic, including its double literals.
+// Speed update, one block per graph. Reference: SpeedKernel's last block,
+// src/RPFA2Kernels.cu:167-219 (Gephi's adaptive speed). The graph's group partials are
+// summed in double in a fixed order (thread-strided, then an LDS tree), the same for a
+// graph alone or inside a batch; the update itself is the reference's float arithmetic,
+// including its double literals. Graph 0 is mirrored into DevState.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_speed(const WalkPart* __restrict__ part, int np, int n, DevState* st) {
-    __shared__ double rs[1024], rt[1024];
-    __shared__ unsigned long long ri[1024];
+__global__ __launch_bounds__(256) void k_speed(const WalkPart* __restrict__ part, const int* __restrict__ grp_first,
+                                               const int* __restrict__ node_off, SegState* __restrict__ seg,
+                                               DevState* st) {
+    __shared__ double rs[256], rt[256];
+    __shared__ unsigned long long ri[256];
     if (st->err != 0) return;
+    const int g = blockIdx.x;
     const int tid = threadIdx.x;
+    const int g0 = grp_first[g], g1 = grp_first[g + 1];
     double s = 0.0, t = 0.0;
     unsigned long long it = 0;
-    for (int k = tid; k < np; k += 1024) {
-        WalkPart p = part[k];
+    for (int k = g0 + tid; k < g1; k += 256) {
+        const WalkPart p = part[k];
         s += (double)p.swing;
         t += (double)p.traction;
         it += p.interactions;
@@ -242,7 +247,7 @@ __global__ __launch_bounds__(1024) void k_speed(const WalkPart* __restrict__ par
     rt[tid] = t;
     ri[tid] = it;
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
+    for (int o = 128; o > 0; o >>= 1) {
         if (tid < o) {
             rs[tid] += rs[tid + o];
             rt[tid] += rt[tid + o];
@@ -251,14 +256,15 @@ __global__ __launch_bounds__(1024) void k_speed(const WalkPart* __restrict__ par
         __syncthreads();
     }
     if (tid == 0) {
+        const int n = node_off[g + 1] - node_off[g];
+        SegState& q = seg[g];
         const double S = rs[0], T = rt[0];
-        st->swing = S;
-        st->traction = T;
-        st->interactions = ri[0];
-        st->interactions_total += ri[0];
+        q.swing = S;
+        q.traction = T;
+        q.interactions = ri[0];
         const float swg = (float)S, etra = (float)T;
-        float gs = st->global_speed, se = st->speed_efficiency;
-        const float jitter = st->jitter_tolerance, ksmax = st->k_s_max;
+        float gs = q.global_speed, se = q.speed_efficiency;
+        const float jitter = q.jitter_tolerance, ksmax = q.k_s_max;
         const float eojt = (float)(0.05 * sqrtf((float)n));
         const float minjt = sqrtf(eojt);
         float jt = jitter * fmaxf(minjt, fminf(ksmax, eojt * etra / powf((float)n, 2.0f)));
@@ -274,27 +280,39 @@ __global__ __launch_bounds__(1024) void k_speed(const WalkPart* __restrict__ par
             se = (float)(se * 1.3);
         }
         gs += fminf(target - gs, (float)0.5 * gs);
-        st->global_speed = gs;
-        st->speed_efficiency = se;
+        q.global_speed = gs;
+        q.speed_efficiency = se;
+        q.iteration += 1;
+        if (g == 0) {
+            st->swing = S;
+            st->traction = T;
+            st->interactions = ri[0];
+            st->global_speed = gs;
+            st->speed_efficiency = se;
+        }
     }
+    if (tid == 0) atomicAdd(&st->interactions_total, ri[0]);
 }
 
 // ---------------------------------------------------------------------------------------
 // Displacement. Reference: DisplacementKernel, src/RPFA2Kernels.cu:223-251:
 // factor = gs/(1.0 + sqrtf(gs*swg)) (double, because of the literal), p += F*factor,
-// F' = F. Also emits bounding-box partials of the new positions for the next tree.
+// F' = F. A single graph also emits bounding-box partials of the new positions for the
+// next tree (a batch recomputes its per-graph boxes instead).
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_displace(float2* __restrict__ pos, const float2* __restrict__ f,
-                                                  float2* __restrict__ fprev, int n, float4* __restrict__ part,
+                                                  float2* __restrict__ fprev, int n, const int* __restrict__ seg_of,
+                                                  const SegState* __restrict__ seg, float4* __restrict__ part,
                                                   DevState* st) {
     __shared__ float4 red[256];
     const int tid = threadIdx.x;
     const bool live = st->err == 0;
-    const float gs = st->global_speed;
+    const float gs0 = seg[0].global_speed;
     float4 acc = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
     for (int i = blockIdx.x * 256 + tid; i < n; i += gridDim.x * 256) {
         float2 p = pos[i];
         if (live) {
+            const float gs = seg_of ? seg[seg_of[i]].global_speed : gs0;
             const float2 F = f[i];
             const float2 P = fprev[i];
             const float dx = F.x - P.x, dy = F.y - P.y;
@@ -309,6 +327,10 @@ __global__ __launch_bounds__(256) void k_displace(float2* __restrict__ pos, cons
         acc.y = fminf(acc.y, p.y);
         acc.z = fmaxf(acc.z, p.x);
         acc.w = fmaxf(acc.w, p.y);
+    }
+    if (part == nullptr) {
+        if (blockIdx.x == 0 && tid == 0 && live) st->iteration += 1;
+        return;
     }
     red[tid] = acc;
     __syncthreads();
@@ -334,9 +356,9 @@ __global__ __launch_bounds__(256) void k_displace(float2* __restrict__ pos, cons
 
 }  // namespace
 
-int walk_blocks(int n, int group) {
-    (void)group;
-    int b = (n + 255) / 256;
+int walk_blocks(int ngroups, int group) {
+    const int per_block = 256 / group;
+    int b = (ngroups + per_block - 1) / per_block;
     return b < 1 ? 1 : b;
 }
 
@@ -350,21 +372,24 @@ void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
 }
 
 void launch_walk(const Bufs& b, const Params& p, hipStream_t s) {
-    const int blocks = walk_blocks(b.n, p.group);
+    const int blocks = walk_blocks(b.ngroups, p.group);
     if (p.group == 64)
-        k_walk<64><<<blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.n, b.fga, b.fprev, b.f, b.rep,
-                                          b.walk_part, p.k_r, p.itolsq, p.epssq, b.st);
+        k_walk<64><<<blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin,
+                                          b.seg_root, b.fga, b.fprev, b.f, b.rep, b.grp_part, p.k_r, p.itolsq,
+                                          p.epssq, b.st);
     else
-        k_walk<32><<<blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.n, b.fga, b.fprev, b.f, b.rep,
-                                          b.walk_part, p.k_r, p.itolsq, p.epssq, b.st);
+        k_walk<32><<<blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin,
+                                          b.seg_root, b.fga, b.fprev, b.f, b.rep, b.grp_part, p.k_r, p.itolsq,
+                                          p.epssq, b.st);
 }
 
 void launch_speed(const Bufs& b, hipStream_t s) {
-    k_speed<<<1, 1024, 0, s>>>(b.walk_part, walk_blocks(b.n, 32), b.n, b.st);
+    k_speed<<<b.nseg, 256, 0, s>>>(b.grp_part, b.grp_first, b.node_off, b.seg_st, b.st);
 }
 
 void launch_displacement(const Bufs& b, hipStream_t s) {
-    k_displace<<<displacement_blocks(b.n), 256, 0, s>>>(b.pos, b.f, b.fprev, b.n, b.bbox_part, b.st);
+    k_displace<<<displacement_blocks(b.n), 256, 0, s>>>(b.pos, b.f, b.fprev, b.n, b.nseg > 1 ? b.seg_of : nullptr,
+                                                       b.seg_st, b.nseg > 1 ? nullptr : b.bbox_part, b.st);
 }
 
 }  // namespace gfa2

@@ -1,6 +1,7 @@
 // C entry points of the host layer (include/gfa2host.h).
 #include "gfa2host.h"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -76,4 +77,60 @@ int gfa2host_write_layout(gfa2host_graph_t h, const float* xy, const char* path,
 }
 
 void gfa2host_free(gfa2host_graph_t h) { delete h; }
+}
+
+namespace
+{
+    struct SplitMix64
+    {
+        uint64_t x;
+        uint64_t next()
+        {
+            uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        }
+        uint64_t below(uint64_t r) { return next() % r; }
+    };
+}
+
+extern "C" int64_t gfa2host_barabasi_albert(int32_t n, int32_t m, uint64_t seed, int32_t* s, int32_t* t)
+{
+    if (m < 1 || n <= m) return 0;
+    SplitMix64 rng{seed};
+    std::vector<int32_t> pool;
+    pool.reserve((size_t)2 * m * n);
+    int64_t e = 0;
+    for (int32_t u = 1; u <= m; ++u)
+    {
+        s[e] = 0;
+        t[e] = u;
+        ++e;
+        pool.push_back(0);
+        pool.push_back(u);
+    }
+    std::vector<int32_t> chosen;
+    chosen.reserve(m);
+    for (int32_t v = m + 1; v < n; ++v)
+    {
+        chosen.clear();
+        while ((int32_t)chosen.size() < m)
+        {
+            const int32_t u = pool[rng.below(pool.size())];
+            bool dup = false;
+            for (int32_t c : chosen) dup |= c == u;
+            if (!dup) chosen.push_back(u);
+        }
+        std::sort(chosen.begin(), chosen.end());
+        for (int32_t u : chosen)
+        {
+            s[e] = v;
+            t[e] = u;
+            ++e;
+            pool.push_back(v);
+            pool.push_back(u);
+        }
+    }
+    return e;
 }

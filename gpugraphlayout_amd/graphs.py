@@ -99,27 +99,16 @@ def write_edgelist(path: str, s, t):
 
 def barabasi_albert(n: int, m: int, seed: int):
     """Preferential attachment from a star K_{1,m}: nodes m+1..n-1 each attach to m
-    distinct earlier nodes chosen with probability proportional to degree.
-    Returns (s, t) in creation order; m*(n-m-1)+m edges."""
-    rng = np.random.default_rng(seed)
-    s = [0] * m
-    t = list(range(1, m + 1))
-    # endpoint multiset: sampling uniformly from it is sampling proportional to degree
-    pool = np.empty(2 * m * n + 2 * m, np.int64)
-    pool[: 2 * m : 2] = 0
-    pool[1 : 2 * m : 2] = np.arange(1, m + 1)
-    size = 2 * m
-    for v in range(m + 1, n):
-        chosen = set()
-        while len(chosen) < m:
-            chosen.add(int(pool[rng.integers(size)]))
-        for u in sorted(chosen):
-            s.append(v)
-            t.append(u)
-            pool[size] = v
-            pool[size + 1] = u
-            size += 2
-    return np.array(s, np.int64), np.array(t, np.int64)
+    distinct earlier nodes chosen with probability proportional to degree (the C++
+    generator of the host library, seeded splitmix64). Returns (s, t) in creation order;
+    m*(n-m-1)+m edges."""
+    from .host import lib as host_lib
+    count = m * (n - m - 1) + m
+    s = np.zeros(max(count, 0), np.int32)
+    t = np.zeros(max(count, 0), np.int32)
+    got = host_lib().gfa2host_barabasi_albert(int(n), int(m), int(seed) & (2 ** 64 - 1),
+                                              s.ctypes.data_as(ctypes.c_void_p), t.ctypes.data_as(ctypes.c_void_p))
+    return s[:got].astype(np.int64), t[:got].astype(np.int64)
 
 
 def rmat(scale: int, n_edges: int, a=0.57, b=0.19, c=0.19, seed: int = 1, batch: int = 1 << 21):

@@ -26,6 +26,8 @@ def lib():
         L.gfa2host_write_layout.argtypes = [_P, _P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
         L.gfa2host_free.argtypes = [_P]
         L.gfa2host_free.restype = None
+        L.gfa2host_barabasi_albert.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, _P, _P]
+        L.gfa2host_barabasi_albert.restype = ctypes.c_int64
         _lib = L
     return _lib
 

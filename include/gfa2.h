@@ -92,6 +92,18 @@ int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_
 int gfa2_set_graph_coo(gfa2_handle h, int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
                        const float* mass);
 
+/* A batch of independent graphs laid out together (no counterpart in the reference, which
+ * holds one graph per process): graph g owns nodes [node_off[g], node_off[g+1]) of a
+ * block-diagonal symmetric CSR (no edge may leave its graph). Each graph gets its own root
+ * box, tree, walk groups and speed state, so it follows the trajectory it would follow
+ * alone. Positions are the concatenation. */
+int gfa2_set_graph_batch(gfa2_handle h, int32_t ngraphs, const int32_t* node_off, const int64_t* rowptr,
+                         const int32_t* col, const float* mass);
+/* Per-graph speed, efficiency, swing, traction and interaction count of the last
+ * iteration (any pointer may be NULL). */
+int gfa2_get_batch_stats(gfa2_handle h, int32_t cap, float* speed, float* efficiency, double* swing,
+                         double* traction, int64_t* interactions);
+
 /* H2D positions (interleaved x,y; n pairs). The reference's private sendLayoutToGPU,
  * src/RPGPUForceAtlas2.cu:256-260, made public. Does not reset F' or the speed state. */
 int gfa2_set_positions(gfa2_handle h, const float* xy);
@@ -137,6 +149,18 @@ int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** nam
 
 /* Graph capture of the whole iteration (hipGraph), replayed per iteration. */
 int gfa2_set_graph_capture(gfa2_handle h, int32_t on);
+
+/* Layout quality (no counterpart in the reference). Samples nsources distinct sources and
+ * targets_per_source targets each (uniform over the other nodes; the engine's own seeded
+ * generator) and computes their hop distances on the device by multi-source BFS. *npairs
+ * receives the number of reachable pairs. */
+int gfa2_stress_setup(gfa2_handle h, int32_t nsources, int32_t targets_per_source, uint64_t seed,
+                      int64_t* npairs);
+/* The sampled pairs: a[p], b[p], hop distance d[p] (0 = unreachable, left out). */
+int gfa2_stress_pairs(gfa2_handle h, int64_t cap, int32_t* a, int32_t* b, int32_t* d);
+/* Normalized stress (1/P) sum (alpha*|p_a-p_b|/d - 1)^2 with the optimal scale alpha, of the
+ * device positions (xy == NULL) or of the given interleaved positions. */
+int gfa2_normalized_stress(gfa2_handle h, const float* xy, double* ns, double* alpha);
 
 int gfa2_destroy(gfa2_handle h);
 

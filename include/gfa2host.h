@@ -32,6 +32,12 @@ int gfa2host_write_layout(gfa2host_graph_t g, const float* xy, const char* path,
                           int32_t image_h);
 void gfa2host_free(gfa2host_graph_t g);
 
+/* Barabasi-Albert preferential attachment, deterministic per seed (splitmix64): a star
+ * K_{1,m} on nodes 0..m, then each node v = m+1..n-1 attaches to m distinct earlier nodes
+ * drawn with probability proportional to degree (uniform draws from the endpoint list).
+ * Writes m*(n-m-1)+m edges (v, u) in creation order, u ascending per v. Returns the count. */
+int64_t gfa2host_barabasi_albert(int32_t n, int32_t m, uint64_t seed, int32_t* s, int32_t* t);
+
 #ifdef __cplusplus
 }
 #endif

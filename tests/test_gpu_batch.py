@@ -1,0 +1,62 @@
+"""A batch of graphs laid out together equals, bit for bit, each graph laid out alone."""
+import numpy as np
+import pytest
+
+from gpugraphlayout_amd import BatchEngine, Engine, graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def batch_case(seed):
+    out = []
+    for k, (n, m) in enumerate([(300, 3), (50, 2), (1000, 5), (2, 1), (120, 4), (7, 1)]):
+        s, t = graphs.barabasi_albert(n, m, seed + k)
+        g = graphs.edges_to_graph(s, t)
+        out.append(g)
+    s, t = graphs.rmat(11, 6000, seed=seed)
+    out.append(graphs.edges_to_graph(s, t))
+    return out
+
+
+@pytest.mark.parametrize("group", [32, 64])
+def test_batch_equals_single_runs(group):
+    print(f"case batch G={group}", flush=True)
+    gs = batch_case(3)
+    pos = [graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 40 + k) for k, g in enumerate(gs)]
+    params = dict(scale=2.0, gravity=1.0, strong_gravity=1, group_size=group)
+    be = BatchEngine([(g.n, g.src, g.dst) for g in gs], **params)
+    be.set_positions(np.concatenate(pos))
+    be.step(25)
+    be.sync()
+    got = be.split(be.positions())
+    st = be.batch_stats()
+    for k, g in enumerate(gs):
+        e = Engine(g.n, g.src, g.dst, **params)
+        e.set_positions(pos[k])
+        e.step(25)
+        e.sync()
+        np.testing.assert_array_equal(got[k].view(np.uint32), e.positions().view(np.uint32))
+        s1 = e.stats()
+        assert st["speed"][k] == np.float32(s1["global_speed"])
+        assert st["swing"][k] == s1["swing"]
+        e.close()
+    be.close()
+
+
+def test_batch_with_single_node_graph():
+    print("case batch single node", flush=True)
+    g1 = graphs.edges_to_graph(*graphs.barabasi_albert(80, 2, 1))
+    be = BatchEngine([(1, np.zeros(0, np.int32), np.zeros(0, np.int32)), (g1.n, g1.src, g1.dst)], scale=2.0)
+    pos = np.concatenate([np.array([[3.0, 4.0]], np.float32), graphs.uniform_positions(g1.n, 18, 2)])
+    be.set_positions(pos)
+    be.step(10)
+    be.sync()
+    p = be.positions()
+    assert np.all(np.isfinite(p))
+    # a lone node only feels gravity: it moves straight toward the origin
+    assert abs(p[0, 0] / p[0, 1] - 0.75) < 1e-5 and np.linalg.norm(p[0]) < 5.0
+    e = Engine(g1.n, g1.src, g1.dst, scale=2.0)
+    e.set_positions(pos[1:])
+    e.step(10)
+    e.sync()
+    np.testing.assert_array_equal(p[1:].view(np.uint32), e.positions().view(np.uint32))
