@@ -337,3 +337,110 @@ class BatchEngine(Engine):
 
     def split(self, xy):
         return [xy[self.node_off[k]:self.node_off[k + 1]] for k in range(self.ngraphs)]
+
+
+MPNN_SYMBOLS = (
+    "gfa2_mpnn_num_params", "gfa2_mpnn_create", "gfa2_mpnn_get_params", "gfa2_mpnn_set_params",
+    "gfa2_mpnn_add_graph", "gfa2_mpnn_features", "gfa2_mpnn_loss_grad", "gfa2_mpnn_train",
+    "gfa2_mpnn_target_scale", "gfa2_mpnn_infer", "gfa2_mpnn_infer_to_engine", "gfa2_mpnn_destroy",
+)
+ABI_SYMBOLS = ABI_SYMBOLS + MPNN_SYMBOLS
+
+
+def _bind_mpnn(L):
+    if getattr(L, "_mpnn_bound", False):
+        return L
+    L.gfa2_mpnn_num_params.argtypes = []
+    L.gfa2_mpnn_num_params.restype = ctypes.c_int32
+    L.gfa2_mpnn_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_P)]
+    L.gfa2_mpnn_get_params.argtypes = [_P, _P]
+    L.gfa2_mpnn_set_params.argtypes = [_P, _P]
+    L.gfa2_mpnn_add_graph.argtypes = [_P, _I32, _P, _P, _P, ctypes.c_uint64, _I32, _P]
+    L.gfa2_mpnn_features.argtypes = [_P, _I32, _P]
+    L.gfa2_mpnn_loss_grad.argtypes = [_P, _I32, _P, _I32, _P, _P]
+    L.gfa2_mpnn_train.argtypes = [_P, _I32, ctypes.c_float, _I32, ctypes.c_uint64, _P]
+    L.gfa2_mpnn_target_scale.argtypes = [_P]
+    L.gfa2_mpnn_target_scale.restype = ctypes.c_float
+    L.gfa2_mpnn_infer.argtypes = [_P, _I32, ctypes.c_float, _P]
+    L.gfa2_mpnn_infer_to_engine.argtypes = [_P, _I32, ctypes.c_float, _P]
+    L.gfa2_mpnn_destroy.argtypes = [_P]
+    for name in MPNN_SYMBOLS:
+        if name not in ("gfa2_mpnn_num_params", "gfa2_mpnn_target_scale"):
+            getattr(L, name).restype = ctypes.c_int
+    L._mpnn_bound = True
+    return L
+
+
+class Mpnn:
+    """The learned initialiser (gfa2_mpnn_*): 2 message-passing layers, hidden 64, BF16
+    weights, FP32 accumulation, trained on the device."""
+
+    IN, HID = 16, 64
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        L = _bind_mpnn(lib())
+        h = ctypes.c_void_p()
+        _check(L.gfa2_mpnn_create(device, int(seed), ctypes.byref(h)), "gfa2_mpnn_create")
+        self.h = h
+        self.nparams = L.gfa2_mpnn_num_params()
+        self.sizes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gfa2_mpnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().gfa2_mpnn_get_params(self.h, _ptr(out)), "gfa2_mpnn_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().gfa2_mpnn_set_params(self.h, _ptr(p)), "gfa2_mpnn_set_params")
+
+    def add_graph(self, n, rowptr, col, target=None, feature_seed: int = 0, steps: int = 8) -> int:
+        rp = np.ascontiguousarray(rowptr, np.int64)
+        c = np.ascontiguousarray(col, np.int32)
+        t = None if target is None else np.ascontiguousarray(target, np.float32)
+        gid = ctypes.c_int32(0)
+        _check(lib().gfa2_mpnn_add_graph(self.h, int(n), _ptr(rp), _ptr(c), _ptr(t), int(feature_seed), int(steps),
+                                         ctypes.byref(gid)), "gfa2_mpnn_add_graph")
+        self.sizes[gid.value] = int(n)
+        return gid.value
+
+    def features(self, gid) -> np.ndarray:
+        out = np.zeros((self.sizes[gid], self.IN), np.float32)
+        _check(lib().gfa2_mpnn_features(self.h, gid, _ptr(out)), "gfa2_mpnn_features")
+        return out
+
+    def loss_grad(self, gid, offsets):
+        o = np.ascontiguousarray(offsets, np.int32)
+        loss = ctypes.c_double(0)
+        g = np.zeros(self.nparams, np.float32)
+        _check(lib().gfa2_mpnn_loss_grad(self.h, gid, _ptr(o), len(o), ctypes.byref(loss), _ptr(g)),
+               "gfa2_mpnn_loss_grad")
+        return loss.value, g
+
+    def train(self, steps, lr=1e-3, R=8, seed=0) -> float:
+        loss = ctypes.c_double(0)
+        _check(lib().gfa2_mpnn_train(self.h, int(steps), float(lr), int(R), int(seed), ctypes.byref(loss)),
+               "gfa2_mpnn_train")
+        return loss.value
+
+    def target_scale(self) -> float:
+        return float(lib().gfa2_mpnn_target_scale(self.h))
+
+    def infer(self, gid, scale) -> np.ndarray:
+        out = np.zeros((self.sizes[gid], 2), np.float32)
+        _check(lib().gfa2_mpnn_infer(self.h, gid, float(scale), _ptr(out)), "gfa2_mpnn_infer")
+        return out
+
+    def infer_to_engine(self, gid, scale, engine: "Engine"):
+        _check(lib().gfa2_mpnn_infer_to_engine(self.h, gid, float(scale), engine.h), "gfa2_mpnn_infer_to_engine")

@@ -777,6 +777,31 @@ int gfa2_normalized_stress(gfa2_handle h, const float* xy, double* ns, double* a
     return GFA2_OK;
 }
 
+}  // extern "C"
+
+namespace gfa2 {
+
+bool engine_view(gfa2_engine* h, EngineView* v) {
+    if (!h || !h->has_graph) return false;
+    v->n = h->b.n;
+    v->device = h->device;
+    v->pos = h->b.pos;
+    v->stream = h->stream;
+    return true;
+}
+
+int engine_positions_written(gfa2_engine* h) {
+    h->has_pos = true;
+    if (h->b.nseg == 1) launch_bbox_partials(h->b, h->stream);
+    GFA2_TRY(hipGetLastError());
+    GFA2_TRY(hipStreamSynchronize(h->stream));
+    return GFA2_OK;
+}
+
+}  // namespace gfa2
+
+extern "C" {
+
 int gfa2_destroy(gfa2_handle h) {
     if (!h) return GFA2_ERR_INVALID_ARG;
     hipSetDevice(h->device);

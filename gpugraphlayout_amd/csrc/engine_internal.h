@@ -22,6 +22,8 @@
 #include <cstdint>
 #include <cstddef>
 
+struct gfa2_engine;
+
 namespace gfa2 {
 
 constexpr int kKeyLevels = 32;          // quadrant digits per key
@@ -159,6 +161,17 @@ int stress_hop_distances(const Bufs& b, const int* d_src, int K, const int* d_tg
                          hipStream_t s);
 int stress_reduce(const float2* pos, const int* d_src, const int* d_tgt, const int* d_dist, int T, long long np,
                   double* d_part, int nblocks, double out[3], hipStream_t s);
+
+// Access to an engine's device positions for code that writes them on the device (the
+// learned initialiser): the view, and the notification that they were written.
+struct EngineView {
+    int n;
+    int device;
+    float2* pos;
+    hipStream_t stream;
+};
+bool engine_view(struct ::gfa2_engine* h, EngineView* v);
+int engine_positions_written(struct ::gfa2_engine* h);
 
 // Walk grid sizes (partials count depends on them).
 int walk_blocks(int ngroups, int group);

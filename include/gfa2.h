@@ -162,6 +162,42 @@ int gfa2_stress_pairs(gfa2_handle h, int64_t cap, int32_t* a, int32_t* b, int32_
  * device positions (xy == NULL) or of the given interleaved positions. */
 int gfa2_normalized_stress(gfa2_handle h, const float* xy, double* ns, double* alpha);
 
+/* ---- Learned initial layout (no counterpart in the reference) ----------------------------
+ * A 2-layer message-passing network (hidden 64, BF16 weights, FP32 accumulation) mapping
+ * per-node features (log degree + 8 seeded random-walk fields) to 2-D positions that
+ * ForceAtlas2 then refines. Trained on the device with its own forward/backward kernels
+ * against layouts FA2 produced, with a stress loss invariant to translation, rotation,
+ * reflection and scale of the target. */
+typedef struct gfa2_mpnn* gfa2_mpnn_t;
+
+int32_t gfa2_mpnn_num_params(void);
+/* Glorot-uniform weights from seed, zero biases. */
+int gfa2_mpnn_create(int device, uint64_t seed, gfa2_mpnn_t* out);
+/* FP32 master parameters (gfa2_mpnn_num_params floats); set resets the optimiser. */
+int gfa2_mpnn_get_params(gfa2_mpnn_t m, float* params);
+int gfa2_mpnn_set_params(gfa2_mpnn_t m, const float* params);
+/* Registers a graph (symmetric CSR) and computes its features on the device
+ * (smoothing_steps rounds of random-walk smoothing from feature_seed). target_xy (may be
+ * NULL) is the layout to learn, interleaved x,y. */
+int gfa2_mpnn_add_graph(gfa2_mpnn_t m, int32_t n, const int64_t* rowptr, const int32_t* col, const float* target_xy,
+                        uint64_t feature_seed, int32_t smoothing_steps, int32_t* graph_id);
+/* Features of a registered graph, [n x 16] (columns 9..15 are zero). */
+int gfa2_mpnn_features(gfa2_mpnn_t m, int32_t graph_id, float* X);
+/* Loss and its gradient (w.r.t. the BF16-rounded parameters) on one graph; pairs are the
+ * CSR neighbours plus i +/- offsets[k] (mod n), k < R. grad may be NULL. */
+int gfa2_mpnn_loss_grad(gfa2_mpnn_t m, int32_t graph_id, const int32_t* offsets, int32_t R, double* loss,
+                        float* grad);
+/* steps Adam steps (lr) cycling over the graphs that have targets; R seeded offsets per step.
+ * *last_loss (may be NULL) receives the last step's loss. */
+int gfa2_mpnn_train(gfa2_mpnn_t m, int32_t steps, float lr, int32_t R, uint64_t seed, double* last_loss);
+/* Mean RMS radius of the training targets (the size the network's output is scaled to). */
+float gfa2_mpnn_target_scale(gfa2_mpnn_t m);
+/* Forward pass, centred and scaled to RMS radius `scale`; to host, or straight into an
+ * engine's device positions (same node count and device). */
+int gfa2_mpnn_infer(gfa2_mpnn_t m, int32_t graph_id, float scale, float* xy);
+int gfa2_mpnn_infer_to_engine(gfa2_mpnn_t m, int32_t graph_id, float scale, gfa2_handle h);
+int gfa2_mpnn_destroy(gfa2_mpnn_t m);
+
 int gfa2_destroy(gfa2_handle h);
 
 const char* gfa2_strerror(int code);

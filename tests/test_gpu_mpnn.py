@@ -1,0 +1,140 @@
+"""The learned initialiser on the device against its FP64 oracle, and what it gains."""
+import numpy as np
+import pytest
+
+from gpugraphlayout_amd import Engine, Mpnn, graphs
+from oracle import mpnn_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def graph_case(seed, n=600):
+    s, t, _ = graphs.sbm(n, 6, 2.0, seed)
+    g = graphs.edges_to_graph(s, t)
+    rowptr, col = g.csr()
+    return g, rowptr, col
+
+
+def normalised(T):
+    T = T.astype(np.float64) - T.mean(0)
+    return T / np.sqrt((T ** 2).sum(1).mean())
+
+
+def test_features_match_oracle():
+    print("case mpnn features", flush=True)
+    g, rowptr, col = graph_case(1)
+    m = Mpnn(seed=0)
+    gid = m.add_graph(g.n, rowptr, col, feature_seed=1234, steps=8)
+    X = m.features(gid)
+    ref = mo.features(rowptr, col, 1234, 8)
+    assert np.max(np.abs(X - ref)) < 1e-4
+    m.close()
+
+
+def test_loss_and_gradient_match_oracle():
+    print("case mpnn gradient", flush=True)
+    g, rowptr, col = graph_case(2)
+    rng = np.random.default_rng(5)
+    T = rng.normal(size=(g.n, 2)).astype(np.float32) * 7 + 3
+    m = Mpnn(seed=3)
+    gid = m.add_graph(g.n, rowptr, col, target=T, feature_seed=9, steps=4)
+    p = m.params()
+    offs = [17, 101, 250]
+    loss, grad = m.loss_grad(gid, offs)
+    X = m.features(gid)
+    L, G = mo.loss_grad(p.astype(np.float64), rowptr, col, X, normalised(T), offs, round_bf16=True)
+    assert loss == pytest.approx(L, rel=1e-5)
+    assert np.linalg.norm(grad - G) / np.linalg.norm(G) < 1e-4
+    m.close()
+
+
+def test_training_reduces_loss_and_is_deterministic():
+    print("case mpnn training", flush=True)
+    losses, params = [], []
+    for rep in range(2):
+        m = Mpnn(seed=11)
+        gids = []
+        for k in range(4):
+            g, rowptr, col = graph_case(20 + k)
+            pos = graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 30 + k)
+            e = Engine(g.n, g.src, g.dst, scale=2.0)
+            e.set_positions(pos)
+            e.step(150)
+            e.sync()
+            gids.append(m.add_graph(g.n, rowptr, col, target=e.positions(), feature_seed=k, steps=8))
+            e.close()
+        first, _ = m.loss_grad(gids[0], [5, 77])
+        m.train(300, lr=3e-3, R=8, seed=1)
+        last, _ = m.loss_grad(gids[0], [5, 77])
+        losses.append((first, last))
+        params.append(m.params())
+        m.close()
+    assert losses[0][1] < 0.7 * losses[0][0]
+    np.testing.assert_array_equal(params[0].view(np.uint32), params[1].view(np.uint32))
+
+
+def test_infer_to_engine_places_positions():
+    print("case mpnn infer", flush=True)
+    g, rowptr, col = graph_case(40)
+    m = Mpnn(seed=2)
+    gid = m.add_graph(g.n, rowptr, col, feature_seed=3)
+    xy = m.infer(gid, 25.0)
+    assert np.allclose(xy.mean(0), 0, atol=1e-3)
+    assert np.sqrt((xy.astype(np.float64) ** 2).sum(1).mean()) == pytest.approx(25.0, rel=1e-4)
+    e = Engine(g.n, g.src, g.dst, scale=2.0)
+    m.infer_to_engine(gid, 25.0, e)
+    np.testing.assert_array_equal(e.positions().view(np.uint32), xy.view(np.uint32))
+    e.step(3)
+    e.sync()
+    assert np.all(np.isfinite(e.positions()))
+    e.close()
+    m.close()
+
+
+def test_learned_start_reaches_plain_stress_sooner():
+    """Trained on FA2 layouts of 12 seeded SBM graphs, the network's start lets FA2 reach the
+    stress a plain run has after 300 iterations in fewer iterations, on held-out graphs."""
+    print("case mpnn gain", flush=True)
+    n, it_plain = 1500, 300
+    m = Mpnn(seed=0)
+    for k in range(12):
+        s, t, _ = graphs.sbm(n, 8, 2.0, 1000 + k)
+        g = graphs.edges_to_graph(s, t)
+        rowptr, col = g.csr()
+        e = Engine(g.n, g.src, g.dst, scale=2.0)
+        e.set_positions(graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), k))
+        e.step(it_plain)
+        e.sync()
+        m.add_graph(g.n, rowptr, col, target=e.positions(), feature_seed=k)
+        e.close()
+    m.train(600, lr=3e-3, R=8, seed=2)
+    scale = m.target_scale()
+    wins = 0
+    for k in range(3):
+        s, t, _ = graphs.sbm(n, 8, 2.0, 5000 + k)
+        g = graphs.edges_to_graph(s, t)
+        rowptr, col = g.csr()
+        plain = Engine(g.n, g.src, g.dst, scale=2.0)
+        plain.set_positions(graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 100 + k))
+        plain.stress_setup(200, 100, seed=k)
+        plain.step(it_plain)
+        plain.sync()
+        ns_plain, _ = plain.normalized_stress()
+        plain.close()
+        gid = m.add_graph(g.n, rowptr, col, feature_seed=100 + k)
+        e = Engine(g.n, g.src, g.dst, scale=2.0)
+        m.infer_to_engine(gid, scale, e)
+        e.stress_setup(200, 100, seed=k)
+        done = None
+        for it in range(0, it_plain + 1, 10):
+            if it:
+                e.step(10)
+            if e.normalized_stress()[0] <= ns_plain:
+                done = it
+                break
+        e.close()
+        print(f"held-out {k}: plain NS {ns_plain:.4f} at {it_plain} it; learned reaches it at {done}", flush=True)
+        if done is not None and done <= it_plain // 2:
+            wins += 1
+    m.close()
+    assert wins >= 2
