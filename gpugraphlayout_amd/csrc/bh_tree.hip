@@ -176,13 +176,65 @@ __device__ int block_exclusive_scan(int v, int* total) {
     return before + inc - v;
 }
 
-// lcp per sorted position and the number of cells opened there; block sums.
+__device__ __forceinline__ Mass3 m3_add(Mass3 a, Mass3 b) { return Mass3{a.m + b.m, a.x + b.x, a.y + b.y}; }
+
+// Block-wide exclusive scan of one Mass3 per thread (NT threads); *total gets the sum.
+template <int NT>
+__device__ Mass3 block_exclusive_scan3(Mass3 v, Mass3* total) {
+    __shared__ double ws[3][NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double a = v.m, b = v.x, c = v.y;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double ta = __shfl_up(a, o, 64), tb = __shfl_up(b, o, 64), tc = __shfl_up(c, o, 64);
+        if (lane >= o) {
+            a += ta;
+            b += tb;
+            c += tc;
+        }
+    }
+    if (lane == 63) {
+        ws[0][wave] = a;
+        ws[1][wave] = b;
+        ws[2][wave] = c;
+    }
+    __syncthreads();
+    Mass3 before{0.0, 0.0, 0.0}, all{0.0, 0.0, 0.0};
+    for (int w = 0; w < NT / 64; ++w) {
+        const Mass3 s{ws[0][w], ws[1][w], ws[2][w]};
+        if (w < wave) before = m3_add(before, s);
+        all = m3_add(all, s);
+    }
+    __syncthreads();
+    *total = all;
+    return Mass3{before.m + a - v.m, before.x + b - v.x, before.y + c - v.y};
+}
+
+// Mass and mass-weighted position (relative to the graph's root centre) of sorted body i.
+__device__ __forceinline__ Mass3 body_m3(int i, const unsigned int* __restrict__ vals,
+                                         const float2* __restrict__ pos, const float* __restrict__ mass,
+                                         const int* __restrict__ sg, const float4* __restrict__ seg_root) {
+    const unsigned int id = vals[i];
+    const float2 p = pos[id];
+    const float4 c = seg_root[sg ? sg[i] : 0];
+    const double m = mass[id];
+    return Mass3{m, m * ((double)p.x - c.x), m * ((double)p.y - c.y)};
+}
+
+// lcp per sorted position and the number of cells opened there; block sums (of the
+// cell counts and, on the integer-mass path, of body mass sums).
 __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* __restrict__ keys,
                                                       const int* __restrict__ sg, int n,
                                                       signed char* __restrict__ lcp,
-                                                      int* __restrict__ blocksum) {
+                                                      int* __restrict__ blocksum,
+                                                      const unsigned int* __restrict__ vals,
+                                                      const float2* __restrict__ pos,
+                                                      const float* __restrict__ mass,
+                                                      const float4* __restrict__ seg_root,
+                                                      Mass3* __restrict__ bmass) {
     const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
     int sum = 0;
+    Mass3 ms{0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < kTreeItems; ++k) {
         int i = i0 + k;
@@ -191,21 +243,41 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
             int l = lcp_at(keys, sg, n, i);
             lcp[i] = (signed char)l;
             sum += max(0, l - prev);
+            if (bmass) ms = m3_add(ms, body_m3(i, vals, pos, mass, sg, seg_root));
         }
     }
     int total;
     block_exclusive_scan(sum, &total);
     if (threadIdx.x == 0) blocksum[blockIdx.x] = total;
+    if (bmass) {
+        Mass3 mt;
+        block_exclusive_scan3<kTreeThreads>(ms, &mt);
+        if (threadIdx.x == 0) bmass[blockIdx.x] = mt;
+    }
 }
 
 // Single block: exclusive scan of block sums (in place), cell total, capacity check,
 // per-iteration counters reset.
 __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum, int nb, int n,
-                                                      int cap_records, int* __restrict__ rec_end, DevState* st) {
+                                                      int cap_records, int* __restrict__ rec_end, DevState* st,
+                                                      Mass3* __restrict__ bmass) {
     __shared__ int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int per = (nb + 1023) / 1024;
     const int b0 = tid * per;
+    if (bmass) {
+        Mass3 loc{0.0, 0.0, 0.0};
+        for (int k = 0; k < per; ++k)
+            if (b0 + k < nb) loc = m3_add(loc, bmass[b0 + k]);
+        Mass3 tot;
+        Mass3 run = block_exclusive_scan3<1024>(loc, &tot);
+        for (int k = 0; k < per; ++k)
+            if (b0 + k < nb) {
+                const Mass3 c = bmass[b0 + k];
+                bmass[b0 + k] = run;
+                run = m3_add(run, c);
+            }
+    }
     int local = 0;
     for (int k = 0; k < per; ++k)
         if (b0 + k < nb) local += blocksum[b0 + k];
@@ -236,6 +308,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum
         *rec_end = n + all;
         if (n + all > cap_records) st->err |= kErrTreeOverflow;
         st->max_level = 0;
+        st->nbig = 0;
     }
     if (tid <= kMaxLevels) {
         st->level_count[tid] = 0;
@@ -252,7 +325,9 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
                                                          const float2* __restrict__ pos,
                                                          const float* __restrict__ mass,
                                                          float4* __restrict__ rec, int* __restrict__ brec,
-                                                         const DevState* __restrict__ st) {
+                                                         const DevState* __restrict__ st,
+                                                         const float4* __restrict__ seg_root,
+                                                         const Mass3* __restrict__ bmass, Mass3* __restrict__ pmass) {
     const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
     int s[kTreeItems];
     int sum = 0;
@@ -265,6 +340,22 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
     int total;
     int run = block_exclusive_scan(sum, &total) + blockoff[blockIdx.x];
     const bool ok = st->err == 0;
+    if (pmass) {
+        // exclusive prefix of body mass sums over sorted positions
+        Mass3 mine{0.0, 0.0, 0.0};
+        for (int k = 0; k < kTreeItems; ++k)
+            if (i0 + k < n) mine = m3_add(mine, body_m3(i0 + k, vals, pos, mass, sg, seg_root));
+        Mass3 tot;
+        Mass3 pre = m3_add(block_exclusive_scan3<kTreeThreads>(mine, &tot), bmass[blockIdx.x]);
+        for (int k = 0; k < kTreeItems; ++k) {
+            const int i = i0 + k;
+            if (i < n) {
+                pmass[i] = pre;
+                pre = m3_add(pre, body_m3(i, vals, pos, mass, sg, seg_root));
+                if (i == n - 1) pmass[n] = pre;
+            }
+        }
+    }
 #pragma unroll
     for (int k = 0; k < kTreeItems; ++k) {
         int i = i0 + k;
@@ -290,7 +381,9 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
                                                         const int* __restrict__ S, float4* __restrict__ rec,
                                                         int* __restrict__ rcount, int* __restrict__ cell_rec,
                                                         unsigned char* __restrict__ cell_lvl, int* __restrict__ rec_begin,
-                                                        DevState* st) {
+                                                        DevState* st, const Mass3* __restrict__ pmass,
+                                                        const float4* __restrict__ seg_root, int* __restrict__ big_rc,
+                                                        unsigned char* __restrict__ big_lvl) {
     __shared__ int hist[kMaxLevels + 1];
     __shared__ int lmax;
     if (threadIdx.x <= kMaxLevels) hist[threadIdx.x] = 0;
@@ -327,7 +420,28 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
                 const int rc = i + ord;
                 const unsigned int skip = (unsigned int)(b + S[b + 1] + 1 - rc);
                 const unsigned int info = (skip << kInfoLevelBits) | (unsigned int)((L > 0 ? L - 1 : 0) & 31);
-                rec[rc].w = __uint_as_float(info);
+                if (pmass) {
+                    // whole-number masses: a cell lighter than 2^24 has this exact float mass
+                    // in any summation order; heavier cells are summed in quadrant order later
+                    const Mass3 a = pmass[i], z = pmass[b + 1];
+                    const double M = z.m - a.m;
+                    if (M < 16777216.0) {
+                        const float4 c = seg_root[gi];
+                        rec[rc] = make_float4((float)((z.x - a.x) / M + c.x), (float)((z.y - a.y) / M + c.y), (float)M,
+                                              __uint_as_float(info));
+                    } else {
+                        rec[rc].w = __uint_as_float(info);
+                        const int slot = atomicAdd(&st->nbig, 1);
+                        if (slot < kMaxBig) {
+                            big_rc[slot] = rc;
+                            big_lvl[slot] = (unsigned char)L;
+                        } else {
+                            st->err |= kErrTreeOverflow;
+                        }
+                    }
+                } else {
+                    rec[rc].w = __uint_as_float(info);
+                }
                 rcount[rc] = b - i + 1;
                 cell_rec[ord] = rc;
                 cell_lvl[ord] = (unsigned char)L;
@@ -404,6 +518,87 @@ __global__ __launch_bounds__(256) void k_summarize(int L, const int* __restrict_
     }
 }
 
+// Heavy cells (mass >= 2^24, listed by k_nodes) of the integer-mass path, in one
+// workgroup: sorted by record index in LDS, then summarised deepest level first, children
+// in quadrant order (mass from 0.0f, centroid (sum m*p)*(1/mass), as
+// src/RPBHKernels.cu:468-494). A heavy child's summary is read from LDS; light children
+// and bodies were written by earlier launches.
+__global__ __launch_bounds__(256) void k_big(float4* __restrict__ rec, const int* __restrict__ big_rc,
+                                             const unsigned char* __restrict__ big_lvl,
+                                             const DevState* __restrict__ st) {
+    __shared__ int brc[kMaxBig];
+    __shared__ int blv[kMaxBig];
+    __shared__ float4 res[kMaxBig];
+    __shared__ int maxl;
+    if (st->err != 0) return;
+    const int nb = min(st->nbig, kMaxBig);
+    if (nb == 0) return;
+    const int tid = threadIdx.x;
+    int P = 1;
+    while (P < nb) P <<= 1;
+    if (tid == 0) maxl = 0;
+    for (int t = tid; t < P; t += 256) {
+        brc[t] = t < nb ? big_rc[t] : 0x7fffffff;
+        blv[t] = t < nb ? (int)big_lvl[t] : -1;
+    }
+    __syncthreads();
+    // bitonic sort by record index
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < P; t += 256) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const bool up = (t & k) == 0;
+                    if ((brc[t] > brc[u]) == up) {
+                        const int a = brc[t], b = blv[t];
+                        brc[t] = brc[u];
+                        blv[t] = blv[u];
+                        brc[u] = a;
+                        blv[u] = b;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int t = tid; t < nb; t += 256) atomicMax(&maxl, blv[t]);
+    __syncthreads();
+    for (int L = maxl; L >= 0; --L) {
+        for (int t = tid; t < nb; t += 256) {
+            if (blv[t] != L) continue;
+            const int rc = brc[t];
+            const int end = min(rc + (int)(__float_as_uint(rec[rc].w) >> kInfoLevelBits), st->nrecords);
+            float cm = 0.0f, px = 0.0f, py = 0.0f;
+            int r = rc + 1;
+            while (r < end) {
+                float4 c = rec[r];
+                const unsigned int ci = __float_as_uint(c.w);
+                if (ci != 0xffffffffu) {
+                    int lo = 0, hi = nb;   // is r heavy?
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (brc[mid] <= r) lo = mid;
+                        else hi = mid;
+                    }
+                    if (brc[lo] == r) c = res[lo];
+                }
+                px = fmaf(c.x, c.z, px);
+                py = fmaf(c.y, c.z, py);
+                cm += c.z;
+                r += ci == 0xffffffffu ? 1 : max((int)(ci >> kInfoLevelBits), 1);
+            }
+            const float inv = 1.0f / cm;
+            res[t] = make_float4(px * inv, py * inv, cm, 0.0f);
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < nb; t += 256) {
+        const float4 v = res[t];
+        rec[brc[t]].x = v.x;
+        rec[brc[t]].y = v.y;
+        rec[brc[t]].z = v.z;
+    }
+}
+
 }  // namespace
 
 int displacement_blocks(int n) {
@@ -425,13 +620,21 @@ void launch_tree(Bufs& b, hipStream_t s) {
     const int n = b.n;
     const int tiles = (n + kTreeTile - 1) / kTreeTile;
     const int* sg = b.nseg > 1 ? b.sg : nullptr;
-    k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum);
-    k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st);
+    Mass3* bm = b.int_mass ? b.bmass : nullptr;
+    Mass3* pm = b.int_mass ? b.pmass : nullptr;
+    k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum, b.sort.vals[0], b.pos, b.mass,
+                                         b.seg_root, bm);
+    k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st, bm);
     k_prefix<<<tiles, kTreeThreads, 0, s>>>(b.lcp, sg, n, b.blocksum, b.S, b.sort.vals[0], b.pos, b.mass, b.rec,
-                                            b.brec, b.st);
+                                            b.brec, b.st, b.seg_root, bm, pm);
     k_nodes<<<(n + kTreeThreads - 1) / kTreeThreads, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, sg, n, b.S, b.rec,
                                                                            b.rcount, b.cell_rec, b.cell_lvl,
-                                                                           b.rec_begin, b.st);
+                                                                           b.rec_begin, b.st, pm, b.seg_root,
+                                                                           b.big_rc, b.big_lvl);
+    if (b.int_mass) {
+        k_big<<<1, 256, 0, s>>>(b.rec, b.big_rc, b.big_lvl, b.st);
+        return;
+    }
     const int max_cells = b.cap_records - n;
     k_levels<<<(max_cells + 255) / 256, 256, 0, s>>>(b.cell_rec, b.cell_lvl, b.level_list, b.st);
     int sum_blocks = (max_cells / 4 + 255) / 256;

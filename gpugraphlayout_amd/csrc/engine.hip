@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -402,6 +403,21 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
         GFA2_TRY(hipMemcpyAsync(so, seg_of.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
     }
     if (h->capture_rep) GFA2_ALLOC(b.rep, n);
+    {
+        double total = 0.0;
+        bool whole = true;
+        for (int v = 0; v < n && whole; ++v) {
+            whole = m[v] >= 0.0f && m[v] < 16777216.0f && m[v] == std::floor(m[v]);
+            total += m[v];
+        }
+        b.int_mass = whole && total < 16777216.0 * 62.0;
+    }
+    if (b.int_mass) {
+        GFA2_ALLOC(b.pmass, (size_t)n + 1);
+        GFA2_ALLOC(b.bmass, tree_tiles);
+        GFA2_ALLOC(b.big_rc, kMaxBig);
+        GFA2_ALLOC(b.big_lvl, kMaxBig);
+    }
 #undef GFA2_ALLOC
     b.nlight = nlight;
     b.nmedium = (int)medium.size();

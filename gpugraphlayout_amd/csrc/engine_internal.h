@@ -48,11 +48,19 @@ struct DevState {
     int err;
     int iteration;
     int nbbox;          // number of valid bounding-box partials
-    int pad1[2];
+    int nbig;           // heavy cells listed for the in-order summary
+    int pad1;
     int level_count[kMaxLevels + 1];
     int level_offset[kMaxLevels + 2];
     int level_cursor[kMaxLevels + 1];
 };
+
+// Prefix sums over sorted bodies of mass and mass-weighted position (relative to the
+// graph's root centre), for the integer-mass summary path.
+struct Mass3 {
+    double m, x, y;
+};
+constexpr int kMaxBig = 2048;   // cells too heavy for an exact prefix-sum mass (>= 2^24)
 
 struct WalkPart {
     float swing, traction;
@@ -119,6 +127,14 @@ struct Bufs {
     int* grp_first = nullptr;         // graph -> first group, nseg + 1
     struct SegState* seg_st = nullptr;
     WalkPart* grp_part = nullptr;     // per group
+    // Integer masses (all whole numbers, total < 2^24 * 62): a cell lighter than 2^24 has
+    // the same float mass in every summation order, so it is summarised from prefix sums;
+    // only heavier cells are summed in quadrant order.
+    bool int_mass = false;
+    Mass3* pmass = nullptr;           // n + 1
+    Mass3* bmass = nullptr;           // per tree tile
+    int* big_rc = nullptr;            // kMaxBig
+    unsigned char* big_lvl = nullptr; // kMaxBig
 };
 
 // Per-graph FA2 state of a batch.

@@ -226,18 +226,21 @@ __global__ __launch_bounds__(256) void k_walk(const float4* __restrict__ rec, co
 // graph alone or inside a batch; the update itself is the reference's float arithmetic,
 // including its double literals. Graph 0 is mirrored into DevState.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_speed(const WalkPart* __restrict__ part, const int* __restrict__ grp_first,
-                                               const int* __restrict__ node_off, SegState* __restrict__ seg,
-                                               DevState* st) {
-    __shared__ double rs[256], rt[256];
-    __shared__ unsigned long long ri[256];
+constexpr int kSpeedThreads = 1024;
+
+__global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restrict__ part,
+                                                         const int* __restrict__ grp_first,
+                                                         const int* __restrict__ node_off, SegState* __restrict__ seg,
+                                                         DevState* st) {
+    __shared__ double rs[kSpeedThreads], rt[kSpeedThreads];
+    __shared__ unsigned long long ri[kSpeedThreads];
     if (st->err != 0) return;
     const int g = blockIdx.x;
     const int tid = threadIdx.x;
     const int g0 = grp_first[g], g1 = grp_first[g + 1];
     double s = 0.0, t = 0.0;
     unsigned long long it = 0;
-    for (int k = g0 + tid; k < g1; k += 256) {
+    for (int k = g0 + tid; k < g1; k += kSpeedThreads) {
         const WalkPart p = part[k];
         s += (double)p.swing;
         t += (double)p.traction;
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(256) void k_speed(const WalkPart* __restrict__ part
     rt[tid] = t;
     ri[tid] = it;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = kSpeedThreads / 2; o > 0; o >>= 1) {
         if (tid < o) {
             rs[tid] += rs[tid + o];
             rt[tid] += rt[tid + o];
@@ -384,7 +387,7 @@ void launch_walk(const Bufs& b, const Params& p, hipStream_t s) {
 }
 
 void launch_speed(const Bufs& b, hipStream_t s) {
-    k_speed<<<b.nseg, 256, 0, s>>>(b.grp_part, b.grp_first, b.node_off, b.seg_st, b.st);
+    k_speed<<<b.nseg, kSpeedThreads, 0, s>>>(b.grp_part, b.grp_first, b.node_off, b.seg_st, b.st);
 }
 
 void launch_displacement(const Bufs& b, hipStream_t s) {

@@ -78,6 +78,36 @@ def test_tree_bit_exact(oracle, kind, seed):
     assert out["stats"]["cells"] == len(ref["level"])
 
 
+@pytest.mark.parametrize("masses", ["heavy_whole", "fractional"])
+def test_tree_bit_exact_custom_masses(oracle, masses):
+    """Whole-number masses whose total exceeds 2^24 (cells above 2^24 are summed in quadrant
+    order, lighter ones from prefix sums) and fractional masses (per-level summaries)."""
+    print(f"case tree masses {masses}", flush=True)
+    g, pos = make_case("ba", 61)
+    rng = np.random.default_rng(3)
+    if masses == "heavy_whole":
+        mass = rng.integers(100_000, 400_000, g.n).astype(np.float32)
+        assert mass.astype(np.float64).sum() > 2 ** 24 * 20
+    else:
+        mass = (rng.random(g.n) * 7 + 0.37).astype(np.float32)
+    eng = Engine(g.n, g.src, g.dst, mass=mass, scale=2.0)
+    eng.set_capture(True)
+    eng.set_positions(pos)
+    eng.step(1)
+    eng.sync()
+    T = eng.tree()
+    rep = eng.forces(GFA2_FORCE_REPULSION)
+    eng.close()
+    ref = oracle.tree(pos, mass)
+    np.testing.assert_array_equal(T["level"], ref["level"])
+    np.testing.assert_array_equal(T["count"], ref["count"])
+    np.testing.assert_array_equal(T["mass"].view(np.uint32), ref["mass"].view(np.uint32))
+    np.testing.assert_array_equal(T["perm"], ref["perm"])
+    assert np.max(np.abs(T["xy"] - ref["xy"])) <= 1e-6 * float(ref["root"][2]) * 2
+    ref_rep, _ = oracle.bh_repulsion(pos, mass, 2.0, group=32)
+    assert rel_l2(rep, ref_rep) <= 1e-4
+
+
 @pytest.mark.parametrize("group", [32, 64])
 @pytest.mark.parametrize("kind,seed", [("ba", 5), ("rmat", 6), ("sbm", 7)])
 def test_forces_first_iteration(oracle, kind, seed, group):
