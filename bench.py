@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--group", type=int, default=32)
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", type=int, default=10,
+                    help="iterations per candidate launch shape before timing (0: default shapes)")
     ap.add_argument("--graph-replay", type=int, default=100,
                     help="extra iterations replayed as one captured hipGraph per iteration (reported apart)")
     args = ap.parse_args()
@@ -106,6 +108,13 @@ def main():
     eng.set_positions(pos)
     eng.step(args.warmup)
     eng.sync()
+    launch = eng.launch_config()
+    tune_s = None
+    if args.autotune > 0:
+        t0 = time.perf_counter()
+        launch = eng.autotune(args.autotune)
+        tune_s = time.perf_counter() - t0
+        print(f"[bench] autotune {launch} in {tune_s:.2f}s", file=sys.stderr, flush=True)
     s0 = eng.stats()
     print(f"[bench] warm-up done ({args.warmup} it), timing {args.steps} it", file=sys.stderr, flush=True)
 
@@ -168,7 +177,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": dict(cfg, group_size=args.group, parallelism="single-gpu"),
+        "config": dict(cfg, group_size=args.group, parallelism="single-gpu",
+                       walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"]),
         "roofline": {
             "bound": "mfma",
             "engine": "fp32 VALU (no MFMA: the walk is not a dense contraction)",
@@ -187,6 +197,7 @@ def main():
         "kernel_groups_ms": groups,
         "hbm_streaming": hbm,
         "graph_replay_iterations_per_s": replay,
+        "autotune_s": tune_s,
         "cells": C,
         "max_level": s1["max_level"],
         "final_swing": final["swing"],

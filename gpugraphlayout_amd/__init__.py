@@ -33,6 +33,7 @@ ABI_SYMBOLS = (
     "gfa2_strerror", "gfa2_device_info", "gfa2_set_params",
     "gfa2_stress_setup", "gfa2_stress_pairs", "gfa2_normalized_stress",
     "gfa2_set_graph_batch", "gfa2_get_batch_stats",
+    "gfa2_autotune", "gfa2_get_launch_config", "gfa2_set_launch_config", "gfa2_tuned_config",
 )
 
 
@@ -70,6 +71,17 @@ class Stats(ctypes.Structure):
         ("root_y", ctypes.c_float),
         ("radius", ctypes.c_float),
         ("error_flags", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [
+        ("walk_block", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32),
+        ("ms_per_iteration", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -124,6 +136,10 @@ def lib() -> ctypes.CDLL:
     L.gfa2_normalized_stress.argtypes = [_P, _P, _P, _P]
     L.gfa2_set_graph_batch.argtypes = [_P, _I32, _P, _P, _P, _P]
     L.gfa2_get_batch_stats.argtypes = [_P, _I32, _P, _P, _P, _P, _P]
+    L.gfa2_autotune.argtypes = [_P, _I32, ctypes.POINTER(LaunchConfig)]
+    L.gfa2_get_launch_config.argtypes = [_P, ctypes.POINTER(LaunchConfig)]
+    L.gfa2_set_launch_config.argtypes = [_P, ctypes.POINTER(LaunchConfig)]
+    L.gfa2_tuned_config.argtypes = [_I32, ctypes.POINTER(LaunchConfig)]
     for name in ABI_SYMBOLS:
         fn = getattr(L, name)
         if name not in ("gfa2_default_params", "gfa2_strerror"):
@@ -277,6 +293,22 @@ class Engine:
         _check(lib().gfa2_normalized_stress(self.h, _ptr(p), ctypes.byref(ns), ctypes.byref(alpha)),
                "gfa2_normalized_stress")
         return ns.value, alpha.value
+
+    def autotune(self, iters: int = 10) -> dict:
+        """Times every launch shape over `iters` iterations and keeps the fastest; the
+        layout state is restored afterwards and no shape changes a result bit."""
+        c = LaunchConfig()
+        _check(lib().gfa2_autotune(self.h, int(iters), ctypes.byref(c)), "gfa2_autotune")
+        return c.as_dict()
+
+    def launch_config(self) -> dict:
+        c = LaunchConfig()
+        _check(lib().gfa2_get_launch_config(self.h, ctypes.byref(c)), "gfa2_get_launch_config")
+        return c.as_dict()
+
+    def set_launch_config(self, walk_block: int, blocks_per_cu: int):
+        c = LaunchConfig(int(walk_block), int(blocks_per_cu), 0.0)
+        _check(lib().gfa2_set_launch_config(self.h, ctypes.byref(c)), "gfa2_set_launch_config")
 
     def set_timing(self, on: bool = True):
         _check(lib().gfa2_set_timing(self.h, int(bool(on))), "gfa2_set_timing")

@@ -601,18 +601,19 @@ __global__ __launch_bounds__(256) void k_big(float4* __restrict__ rec, const int
 
 }  // namespace
 
-int displacement_blocks(int n) {
-    int b = (n + 255) / 256;
-    return b < 1 ? 1 : (b > kPartBlocks ? kPartBlocks : b);
+int displacement_blocks(const Bufs& b) {
+    int cap = b.grid_cap < kPartBlocks ? b.grid_cap : kPartBlocks;
+    int g = (b.n + 255) / 256;
+    return g < 1 ? 1 : (g > cap ? cap : g);
 }
 
 void launch_bbox_partials(const Bufs& b, hipStream_t s) {
-    k_bbox_partials<<<displacement_blocks(b.n), 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st);
+    k_bbox_partials<<<displacement_blocks(b), 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st);
 }
 
 void launch_keys(const Bufs& b, hipStream_t s) {
     int blocks = (b.n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
+    if (blocks > b.grid_cap) blocks = b.grid_cap;
     k_keys<<<blocks, 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st, b.seg_root, b.sort.keys[0], b.sort.vals[0]);
 }
 

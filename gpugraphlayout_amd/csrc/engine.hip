@@ -9,6 +9,8 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -53,7 +55,29 @@ struct gfa2_engine {
     // graphs of a batch (one for a single graph)
     std::vector<int> node_off_host;
     int groups_cap = 0;
+    // launch shapes (autotuned per graph-size bucket)
+    int cus = 256;
+    LaunchConfig tune{256, 8};
 };
+
+namespace {
+
+// Process-wide autotuning results: graph-size bucket floor(log2 V) -> launch shapes.
+std::mutex g_tune_mu;
+std::map<int, LaunchConfig> g_tune_table;
+
+int size_bucket(int n) {
+    int b = 0;
+    while ((2ll << b) <= n) ++b;
+    return b;
+}
+
+void apply_tune(gfa2_engine* h) {
+    h->p.walk_bt = h->tune.walk_bt;
+    h->b.grid_cap = h->tune.blocks_per_cu * h->cus;
+}
+
+}  // namespace
 
 namespace {
 
@@ -245,6 +269,7 @@ int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out) {
     h->p.itolsq = 1.0f / (prm.theta * prm.theta);
     h->p.epssq = prm.epssq;
     h->p.group = prm.group_size;
+    h->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete h;
@@ -433,6 +458,12 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     h->edges = nnz / 2;
     h->has_graph = true;
     h->has_pos = false;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune_table.find(size_bucket(n));
+        if (it != g_tune_table.end()) h->tune = it->second;
+    }
+    apply_tune(h);
     st = build_groups(h);
     if (st != GFA2_OK) return st;
     return reset_state(h);
@@ -692,6 +723,141 @@ int gfa2_set_graph_capture(gfa2_handle h, int32_t on) {
     h->graph_valid = false;
     if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+    return GFA2_OK;
+}
+
+int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c) {
+    if (!h || !c) return GFA2_ERR_INVALID_ARG;
+    c->walk_block = h->tune.walk_bt;
+    c->blocks_per_cu = h->tune.blocks_per_cu;
+    c->ms_per_iteration = 0.0;
+    return GFA2_OK;
+}
+
+int gfa2_set_launch_config(gfa2_handle h, const gfa2_launch_config* c) {
+    if (!h || !c) return GFA2_ERR_INVALID_ARG;
+    const int bt = c->walk_block;
+    if ((bt != 64 && bt != 128 && bt != 256 && bt != 512) || c->blocks_per_cu < 1 || c->blocks_per_cu > 64)
+        return GFA2_ERR_INVALID_ARG;
+    h->tune = LaunchConfig{bt, c->blocks_per_cu};
+    apply_tune(h);
+    h->graph_valid = false;
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    return GFA2_OK;
+}
+
+int gfa2_tuned_config(int32_t nodes, gfa2_launch_config* c) {
+    if (!c || nodes < 1) return GFA2_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune_table.find(size_bucket(nodes));
+    if (it == g_tune_table.end()) return GFA2_ERR_NO_GRAPH;
+    c->walk_block = it->second.walk_bt;
+    c->blocks_per_cu = it->second.blocks_per_cu;
+    c->ms_per_iteration = 0.0;
+    return GFA2_OK;
+}
+
+// Times every candidate launch shape over `iters` real iterations (HIP events on the
+// handle's stream), keeps the fastest, and restores the layout state so the caller's
+// trajectory is the one it would have been without tuning (no candidate changes a result
+// bit, so only the advanced iterations need undoing).
+int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
+    if (!h || iters < 1) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph || !h->has_pos) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    int rc = drain_timing(h);
+    if (rc != GFA2_OK) return rc;
+    Bufs& b = h->b;
+    const size_t n = (size_t)b.n;
+    // snapshot
+    float2 *spos = nullptr, *sfp = nullptr, *sf = nullptr, *sfga = nullptr;
+    DevState* sst = nullptr;
+    SegState* sseg = nullptr;
+    float4* spart = nullptr;
+    GFA2_TRY(hipMalloc(&spos, sizeof(float2) * n));
+    GFA2_TRY(hipMalloc(&sfp, sizeof(float2) * n));
+    GFA2_TRY(hipMalloc(&sf, sizeof(float2) * n));
+    GFA2_TRY(hipMalloc(&sfga, sizeof(float2) * n));
+    GFA2_TRY(hipMalloc(&sst, sizeof(DevState)));
+    GFA2_TRY(hipMalloc(&sseg, sizeof(SegState) * b.nseg));
+    GFA2_TRY(hipMalloc(&spart, sizeof(float4) * kPartBlocks));
+    auto copy_all = [&](bool save) -> hipError_t {
+        hipError_t e = hipSuccess;
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream);
+        };
+        if (save) {
+            cp(spos, b.pos, sizeof(float2) * n);
+            cp(sfp, b.fprev, sizeof(float2) * n);
+            cp(sf, b.f, sizeof(float2) * n);
+            cp(sfga, b.fga, sizeof(float2) * n);
+            cp(sst, b.st, sizeof(DevState));
+            cp(sseg, b.seg_st, sizeof(SegState) * b.nseg);
+            cp(spart, b.bbox_part, sizeof(float4) * kPartBlocks);
+        } else {
+            cp(b.pos, spos, sizeof(float2) * n);
+            cp(b.fprev, sfp, sizeof(float2) * n);
+            cp(b.f, sf, sizeof(float2) * n);
+            cp(b.fga, sfga, sizeof(float2) * n);
+            cp(b.st, sst, sizeof(DevState));
+            cp(b.seg_st, sseg, sizeof(SegState) * b.nseg);
+            cp(b.bbox_part, spart, sizeof(float4) * kPartBlocks);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        return e;
+    };
+    hipError_t e = copy_all(true);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    const LaunchConfig keep = h->tune;
+    LaunchConfig win = keep;
+    float win_ms = 1e30f;
+    const int bts[] = {64, 128, 256, 512};
+    const int bpcs[] = {1, 2, 4, 8};
+    for (int bt : bts) {
+        for (int bpc : bpcs) {
+            if (e != hipSuccess) break;
+            h->tune = LaunchConfig{bt, bpc};
+            apply_tune(h);
+            enqueue_iteration(h, nullptr);   // warm
+            e = hipEventRecord(t0, h->stream);
+            for (int i = 0; i < iters && e == hipSuccess; ++i) enqueue_iteration(h, nullptr);
+            if (e == hipSuccess) e = hipEventRecord(t1, h->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(t1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            if (e == hipSuccess && ms < win_ms) {
+                win_ms = ms;
+                win = h->tune;
+            }
+            if (e == hipSuccess) e = copy_all(false);
+        }
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    for (void* p : {(void*)spos, (void*)sfp, (void*)sf, (void*)sfga, (void*)sst, (void*)sseg, (void*)spart})
+        (void)hipFree(p);
+    if (e != hipSuccess) {
+        h->tune = keep;
+        apply_tune(h);
+        return hip_status(e);
+    }
+    h->tune = win;
+    apply_tune(h);
+    h->graph_valid = false;
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tune_table[size_bucket(b.n)] = win;
+    }
+    if (best) {
+        best->walk_block = win.walk_bt;
+        best->blocks_per_cu = win.blocks_per_cu;
+        best->ms_per_iteration = win_ms / iters;
+    }
     return GFA2_OK;
 }
 

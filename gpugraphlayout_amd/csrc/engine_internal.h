@@ -80,12 +80,20 @@ struct Params {
     float k_g, k_r;
     float itolsq, epssq;
     int group;              // walk vote group: 32 or 64 lanes
+    int walk_bt = 256;      // walk threads per block (64..512); results do not depend on it
+};
+
+// Launch shapes the autotuner chooses from; none of them changes any result bit.
+struct LaunchConfig {
+    int walk_bt;            // 64, 128, 256, 512
+    int blocks_per_cu;      // grid of the grid-stride kernels (keys, displacement, bbox)
 };
 
 // All device buffers of one instance.
 struct Bufs {
     int n;                  // V
     long long nnz;          // 2E
+    int grid_cap = 2048;    // block cap of the grid-stride kernels (blocks_per_cu * CUs)
     int cap_records;        // capacity of rec / rcount (V + max cells)
     float2* pos;
     float* mass;
@@ -190,7 +198,7 @@ bool engine_view(struct ::gfa2_engine* h, EngineView* v);
 int engine_positions_written(struct ::gfa2_engine* h);
 
 // Walk grid sizes (partials count depends on them).
-int walk_blocks(int ngroups, int group);
-int displacement_blocks(int n);
+int walk_blocks(int ngroups, int group, int bt);
+int displacement_blocks(const Bufs& b);
 
 }  // namespace gfa2

@@ -119,8 +119,8 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
 // The epilogue adds gravity + attraction, writes the total force, and forms the group's
 // swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
 // ---------------------------------------------------------------------------------------
-template <int G>
-__global__ __launch_bounds__(256) void k_walk(const float4* __restrict__ rec, const int* __restrict__ brec,
+template <int G, int BT>
+__global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, const int* __restrict__ brec,
                                               const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
                                               int ngroups, const int* __restrict__ node_off,
                                               const int* __restrict__ rec_begin, const float4* __restrict__ seg_root,
@@ -139,8 +139,8 @@ __global__ __launch_bounds__(256) void k_walk(const float4* __restrict__ rec, co
     int lb;
     if (full == 0) lb = xcd * per + (blockIdx.x >> 3);
     else lb = (xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1)) + (blockIdx.x >> 3);
-    const int gidx = (lb * 256 + tid) / G;
-    const int gl = (lb * 256 + tid) % G;          // lane within the group
+    const int gidx = (lb * BT + tid) / G;
+    const int gl = (lb * BT + tid) % G;           // lane within the group
     const bool group_ok = gidx < ngroups;
     int2 grp = make_int2(0, 0);
     if (group_ok) grp = groups[gidx];
@@ -359,11 +359,32 @@ __global__ __launch_bounds__(256) void k_displace(float2* __restrict__ pos, cons
 
 }  // namespace
 
-int walk_blocks(int ngroups, int group) {
-    const int per_block = 256 / group;
+int walk_blocks(int ngroups, int group, int bt) {
+    const int per_block = bt / group;
     int b = (ngroups + per_block - 1) / per_block;
     return b < 1 ? 1 : b;
 }
+
+namespace {
+
+template <int G, int BT>
+void walk_launch(const Bufs& b, const Params& p, hipStream_t s) {
+    k_walk<G, BT><<<walk_blocks(b.ngroups, G, BT), BT, 0, s>>>(
+        b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev, b.f,
+        b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
+}
+
+template <int G>
+void walk_dispatch(const Bufs& b, const Params& p, hipStream_t s) {
+    switch (p.walk_bt) {
+        case 64: walk_launch<G, 64>(b, p, s); break;
+        case 128: walk_launch<G, 128>(b, p, s); break;
+        case 512: walk_launch<G, 512>(b, p, s); break;
+        default: walk_launch<G, 256>(b, p, s); break;
+    }
+}
+
+}  // namespace
 
 void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
     const int lb = (b.nlight + 255) / 256;
@@ -375,15 +396,8 @@ void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
 }
 
 void launch_walk(const Bufs& b, const Params& p, hipStream_t s) {
-    const int blocks = walk_blocks(b.ngroups, p.group);
-    if (p.group == 64)
-        k_walk<64><<<blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin,
-                                          b.seg_root, b.fga, b.fprev, b.f, b.rep, b.grp_part, p.k_r, p.itolsq,
-                                          p.epssq, b.st);
-    else
-        k_walk<32><<<blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin,
-                                          b.seg_root, b.fga, b.fprev, b.f, b.rep, b.grp_part, p.k_r, p.itolsq,
-                                          p.epssq, b.st);
+    if (p.group == 64) walk_dispatch<64>(b, p, s);
+    else walk_dispatch<32>(b, p, s);
 }
 
 void launch_speed(const Bufs& b, hipStream_t s) {
@@ -391,7 +405,7 @@ void launch_speed(const Bufs& b, hipStream_t s) {
 }
 
 void launch_displacement(const Bufs& b, hipStream_t s) {
-    k_displace<<<displacement_blocks(b.n), 256, 0, s>>>(b.pos, b.f, b.fprev, b.n, b.nseg > 1 ? b.seg_of : nullptr,
+    k_displace<<<displacement_blocks(b), 256, 0, s>>>(b.pos, b.f, b.fprev, b.n, b.nseg > 1 ? b.seg_of : nullptr,
                                                        b.seg_st, b.nseg > 1 ? nullptr : b.bbox_part, b.st);
 }
 

@@ -147,6 +147,25 @@ int gfa2_set_timing(gfa2_handle h, int32_t on);
  * names[i] (static strings), ms[i]; *count receives the number of groups (<= cap). */
 int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** names, double* ms);
 
+/* Launch shapes (replaces the compile-time THREADSn/FACTORn of
+ * src/RPBHFA2LaunchParameters.cuh:10-59). None of them changes any result bit: the walk's
+ * vote group stays 32 or 64 lanes whatever its block size, and the grid-stride kernels
+ * only produce exact min/max partials. */
+typedef struct gfa2_launch_config {
+    int32_t walk_block;         /* walk threads per block: 64, 128, 256 or 512 */
+    int32_t blocks_per_cu;      /* grid of the grid-stride kernels, per compute unit */
+    double ms_per_iteration;    /* measured by gfa2_autotune */
+} gfa2_launch_config;
+
+/* Times every launch shape over `iters` iterations, keeps the fastest for this handle and
+ * for every later handle whose node count falls in the same power-of-two bucket, and
+ * restores the layout state (the run continues as if no tuning iterations were made). */
+int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best);
+int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c);
+int gfa2_set_launch_config(gfa2_handle h, const gfa2_launch_config* c);
+/* The tuned shapes recorded for the size bucket of `nodes`; GFA2_ERR_NO_GRAPH if none. */
+int gfa2_tuned_config(int32_t nodes, gfa2_launch_config* c);
+
 /* Graph capture of the whole iteration (hipGraph), replayed per iteration. */
 int gfa2_set_graph_capture(gfa2_handle h, int32_t on);
 

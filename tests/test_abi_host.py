@@ -41,6 +41,11 @@ def test_library_host_only_calls():
     bad = g2.default_params(group_size=48)
     assert lib.gfa2_create(ctypes.byref(bad), 0, ctypes.byref(h)) != 0
     assert lib.gfa2_step(None, 1) == -3
+    c = g2.LaunchConfig()
+    assert lib.gfa2_tuned_config(0, ctypes.byref(c)) == -3
+    assert lib.gfa2_tuned_config(1000, ctypes.byref(c)) == -7   # nothing tuned in this process
+    assert lib.gfa2_autotune(None, 3, ctypes.byref(c)) == -3
+    assert lib.gfa2_set_launch_config(None, ctypes.byref(c)) == -3
 
 
 def test_ba_generator_invariants():

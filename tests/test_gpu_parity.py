@@ -204,3 +204,50 @@ def test_coincident_bodies_are_finite():
     assert np.all(np.isfinite(eng.forces(GFA2_FORCE_REPULSION)))
     assert sorted(T["perm"].tolist()) == list(range(g.n))
     eng.close()
+
+
+@pytest.mark.parametrize("walk_block,blocks_per_cu", [(64, 1), (128, 2), (512, 8), (256, 3)])
+def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu):
+    print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu}", flush=True)
+    g, pos = make_case("rmat", 43)
+    res = []
+    for shape in (None, (walk_block, blocks_per_cu)):
+        eng = Engine(g.n, g.src, g.dst, scale=10.0)
+        if shape is not None:
+            eng.set_launch_config(*shape)
+            assert eng.launch_config()["walk_block"] == walk_block
+        eng.set_positions(pos)
+        eng.step(20)
+        eng.sync()
+        res.append((eng.positions(), eng.stats()))
+        eng.close()
+    np.testing.assert_array_equal(res[0][0].view(np.uint32), res[1][0].view(np.uint32))
+    assert res[0][1]["global_speed"] == res[1][1]["global_speed"]
+    assert res[0][1]["interactions_total"] == res[1][1]["interactions_total"]
+
+
+def test_autotune_restores_the_trajectory():
+    print("case autotune", flush=True)
+    g, pos = make_case("ba", 45)
+    ref = Engine(g.n, g.src, g.dst, scale=2.0)
+    ref.set_positions(pos)
+    ref.step(20)
+    ref.sync()
+    want = ref.positions()
+    want_stats = ref.stats()
+    ref.close()
+    eng = Engine(g.n, g.src, g.dst, scale=2.0)
+    eng.set_graph_capture(True)
+    eng.set_positions(pos)
+    eng.step(5)
+    best = eng.autotune(3)
+    assert best["walk_block"] in (64, 128, 256, 512) and best["ms_per_iteration"] > 0
+    assert eng.launch_config()["walk_block"] == best["walk_block"]
+    assert eng.stats()["iteration"] == 5
+    eng.step(15)
+    eng.sync()
+    np.testing.assert_array_equal(eng.positions().view(np.uint32), want.view(np.uint32))
+    st = eng.stats()
+    assert st["iteration"] == want_stats["iteration"]
+    assert st["interactions_total"] == want_stats["interactions_total"]
+    eng.close()
