@@ -437,7 +437,7 @@ class Mpnn:
         p = np.ascontiguousarray(p, np.float32)
         _check(lib().gfa2_mpnn_set_params(self.h, _ptr(p)), "gfa2_mpnn_set_params")
 
-    def add_graph(self, n, rowptr, col, target=None, feature_seed: int = 0, steps: int = 8) -> int:
+    def add_graph(self, n, rowptr, col, target=None, feature_seed: int = 0, steps: int = 2) -> int:
         rp = np.ascontiguousarray(rowptr, np.int64)
         c = np.ascontiguousarray(col, np.int32)
         t = None if target is None else np.ascontiguousarray(target, np.float32)

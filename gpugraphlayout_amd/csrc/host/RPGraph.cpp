@@ -1,26 +1,46 @@
 #include "RPGraph.hpp"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
-#include <unordered_set>
+
+#include "hashing.hpp"
 
 namespace RPGraph
 {
     namespace
     {
-        // Parses "s t" (whitespace separated unsigned ids) at the start of a line.
-        bool parse_pair(const char* p, nid_t& s, nid_t& t)
+        inline bool is_digit(char c) { return c >= '0' && c <= '9'; }
+
+        // Unsigned decimal at p (a digit), saturating like strtoul; advances p.
+        inline uint64_t parse_uint(const char*& p, const char* end)
         {
-            char* end = nullptr;
-            while (*p == ' ' || *p == '\t') ++p;
-            if (*p < '0' || *p > '9') return false;
-            unsigned long a = strtoul(p, &end, 10);
-            p = end;
-            while (*p == ' ' || *p == '\t' || *p == ',') ++p;
-            if (*p < '0' || *p > '9') return false;
-            unsigned long b = strtoul(p, &end, 10);
+            uint64_t v = 0;
+            bool sat = false;
+            for (; p < end && is_digit(*p); ++p)
+            {
+                const uint64_t d = (uint64_t)(*p - '0');
+                if (v > (UINT64_MAX - d) / 10) sat = true;
+                else v = v * 10 + d;
+            }
+            return sat ? UINT64_MAX : v;
+        }
+
+        // Parses "s t" (ids separated by spaces, tabs or commas) at the start of the line
+        // [p, end); anything after the second id is ignored.
+        bool parse_pair(const char* p, const char* end, nid_t& s, nid_t& t)
+        {
+            while (p < end && (*p == ' ' || *p == '\t')) ++p;
+            if (p == end || !is_digit(*p)) return false;
+            const uint64_t a = parse_uint(p, end);
+            while (p < end && (*p == ' ' || *p == '\t' || *p == ',')) ++p;
+            if (p == end || !is_digit(*p)) return false;
+            const uint64_t b = parse_uint(p, end);
             s = (nid_t)a;
             t = (nid_t)b;
             return true;
@@ -36,83 +56,119 @@ namespace RPGraph
             loaded = load_binary(path);
             return;
         }
-        FILE* fh = fopen(path.c_str(), "r");
-        if (!fh) return;
-        std::vector<nid_t> s, t;
-        char* line = nullptr;
-        size_t cap = 0;
-        ssize_t len;
-        while ((len = getline(&line, &cap, fh)) != -1)
+        const int fd = open(path.c_str(), O_RDONLY);
+        if (fd < 0) return;
+        struct stat sb;
+        if (fstat(fd, &sb) != 0)
         {
-            if (len == 0 || line[0] == '#') continue;   // comments (src/RPGraph.cpp:46)
-            nid_t a, b;
-            if (!parse_pair(line, a, b)) continue;
-            s.push_back(a);
-            t.push_back(b);
+            close(fd);
+            return;
         }
-        free(line);
-        fclose(fh);
-        build(s, t);
+        const size_t size = (size_t)sb.st_size;
+        std::vector<nid_t> s, t;
+        if (size > 0)
+        {
+            void* map = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (map == MAP_FAILED)
+            {
+                close(fd);
+                return;
+            }
+            madvise(map, size, MADV_SEQUENTIAL);
+            const char* p = static_cast<const char*>(map);
+            const char* end = p + size;
+            s.reserve(size / 16);
+            t.reserve(size / 16);
+            while (p < end)
+            {
+                const char* nl = static_cast<const char*>(memchr(p, '\n', (size_t)(end - p)));
+                const char* le = nl ? nl : end;
+                nid_t a, b;
+                // comments (src/RPGraph.cpp:46)
+                if (le > p && p[0] != '#' && parse_pair(p, le, a, b))
+                {
+                    s.push_back(a);
+                    t.push_back(b);
+                }
+                p = nl ? nl + 1 : end;
+            }
+            munmap(map, size);
+        }
+        close(fd);
+        build(s.data(), t.data(), s.size());
         loaded = true;
     }
 
     UGraph::UGraph(const std::vector<nid_t>& s, const std::vector<nid_t>& t)
     {
-        build(s, t);
+        build(s.data(), t.data(), s.size() < t.size() ? s.size() : t.size());
         loaded = true;
     }
 
-    void UGraph::build(const std::vector<nid_t>& s, const std::vector<nid_t>& t)
+    void UGraph::build(const nid_t* s, const nid_t* t, size_t count)
     {
-        std::unordered_set<uint64_t> seen;
-        seen.reserve(s.size() * 2);
-        node_map.reserve(s.size());
-        for (size_t k = 0; k < s.size(); ++k)
+        gfa2host::KeySet seen(count);
+        gfa2host::IdMap ids(count / 4 + 1024);
+        std::vector<nid_t> lo, hi;
+        lo.reserve(count);
+        hi.reserve(count);
+        constexpr size_t kAhead = 16;
+        for (size_t k = 0; k < count; ++k)
         {
+            if (k + kAhead < count)
+            {
+                const nid_t pa = s[k + kAhead], pb = t[k + kAhead];
+                seen.prefetch(pa < pb ? ((uint64_t)pa << 32 | pb) : ((uint64_t)pb << 32 | pa));
+                ids.prefetch(pa);
+                ids.prefetch(pb);
+            }
             const nid_t a = s[k], b = t[k];
             if (a == b) continue;                                  // self-loop
             const uint64_t key = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
-            if (!seen.insert(key).second) continue;                // repeated edge
-            nid_t ma, mb;
-            auto ia = node_map.find(a);
-            if (ia == node_map.end())
-            {
-                ma = node_count++;
-                node_map.emplace(a, ma);
-                node_map_r.push_back(a);
-            }
-            else ma = ia->second;
-            auto ib = node_map.find(b);
-            if (ib == node_map.end())
-            {
-                mb = node_count++;
-                node_map.emplace(b, mb);
-                node_map_r.push_back(b);
-            }
-            else mb = ib->second;
-            if (adjacency.size() < node_count)
-            {
-                adjacency.resize(node_count);
-                degrees.resize(node_count, 0);
-            }
-            adjacency[ma < mb ? ma : mb].push_back(ma < mb ? mb : ma);
-            degrees[ma] += 1;
-            degrees[mb] += 1;
-            ++edge_count;
+            if (!seen.insert(key)) continue;                       // repeated edge
+            const uint32_t before = ids.size();
+            const nid_t ma = ids.get_or_add(a);
+            if (ids.size() != before) node_map_r.push_back(a);
+            const uint32_t mid = ids.size();
+            const nid_t mb = ids.get_or_add(b);
+            if (ids.size() != mid) node_map_r.push_back(b);
+            lo.push_back(ma < mb ? ma : mb);
+            hi.push_back(ma < mb ? mb : ma);
         }
+        node_count = (nid_t)node_map_r.size();
+        node_map.reserve(node_count);
+        for (nid_t v = 0; v < node_count; ++v) node_map.emplace(node_map_r[v], v);
+        finish(lo, hi);
+    }
+
+    // CSR of the kept edges at their smaller id, in the order given (a stable counting sort).
+    void UGraph::finish(const std::vector<nid_t>& lo, const std::vector<nid_t>& hi)
+    {
+        const size_t m = lo.size();
+        edge_count = (nid_t)m;
+        degrees.assign(node_count, 0);
+        adj_off.assign((size_t)node_count + 1, 0);
+        for (size_t e = 0; e < m; ++e)
+        {
+            degrees[lo[e]] += 1;
+            degrees[hi[e]] += 1;
+            adj_off[lo[e] + 1] += 1;
+        }
+        for (size_t v = 0; v < node_count; ++v) adj_off[v + 1] += adj_off[v];
+        adj.resize(m);
+        std::vector<uint64_t> cur(adj_off.begin(), adj_off.end() - 1);
+        for (size_t e = 0; e < m; ++e) adj[cur[lo[e]]++] = hi[e];
     }
 
     void UGraph::coo(std::vector<int32_t>& src, std::vector<int32_t>& dst) const
     {
-        src.clear();
-        dst.clear();
-        src.reserve(edge_count);
-        dst.reserve(edge_count);
+        src.resize(edge_count);
+        dst.resize(edge_count);
         for (nid_t v = 0; v < node_count; ++v)
-            for (nid_t u : adjacency[v])
+            for (uint64_t e = adj_off[v]; e < adj_off[v + 1]; ++e)
             {
-                src.push_back((int32_t)v);
-                dst.push_back((int32_t)u);
+                src[e] = (int32_t)v;
+                dst[e] = (int32_t)adj[e];
             }
     }
 
@@ -149,18 +205,17 @@ namespace RPGraph
         }
         fclose(fh);
         if (!ok) return false;
-        node_count = (nid_t)n;
-        edge_count = (nid_t)m;
-        adjacency.assign(n, {});
-        degrees.assign(n, 0);
-        for (nid_t v = 0; v < node_count; ++v) node_map.emplace(node_map_r[v], v);
+        // The cached COO is ascending by source with src < dst; anything else is corrupt.
         for (uint64_t e = 0; e < m; ++e)
         {
             if (s[e] < 0 || d[e] < 0 || (uint64_t)s[e] >= n || (uint64_t)d[e] >= n || s[e] >= d[e]) return false;
-            adjacency[s[e]].push_back((nid_t)d[e]);
-            degrees[s[e]] += 1;
-            degrees[d[e]] += 1;
+            if (e > 0 && s[e] < s[e - 1]) return false;
         }
+        node_count = (nid_t)n;
+        node_map.reserve(n);
+        for (nid_t v = 0; v < node_count; ++v) node_map.emplace(node_map_r[v], v);
+        std::vector<nid_t> lo(s.begin(), s.end()), hi(d.begin(), d.end());
+        finish(lo, hi);
         return true;
     }
 }

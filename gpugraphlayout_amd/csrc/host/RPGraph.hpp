@@ -3,7 +3,8 @@
 // repeated undirected edges dropped, node ids numbered 0.. by first appearance (source
 // before target), each edge kept once at its smaller internal id in file order. The
 // reference checks duplicates by copying and scanning a neighbour vector per line
-// (O(sum deg^2)); here a hash set of endpoint pairs makes loading O(E).
+// (O(sum deg^2)); here an open-addressing set of endpoint pairs makes loading O(E), the
+// file is memory-mapped and parsed in place, and adjacency is one CSR array.
 #ifndef RPGraph_hpp
 #define RPGraph_hpp
 #include <cstdint>
@@ -15,6 +16,17 @@ namespace RPGraph
 {
     typedef uint32_t nid_t;   // node ids (src/RPGraph.hpp:34-36)
     typedef uint32_t eid_t;
+
+    // Neighbour list view (a range over the CSR adjacency).
+    struct NeighborRange
+    {
+        const nid_t* b;
+        const nid_t* e;
+        const nid_t* begin() const { return b; }
+        const nid_t* end() const { return e; }
+        size_t size() const { return (size_t)(e - b); }
+        nid_t operator[](size_t i) const { return b[i]; }
+    };
 
     class UGraph
     {
@@ -31,9 +43,12 @@ namespace RPGraph
         nid_t degree(nid_t nid) const { return degrees[nid]; }
         nid_t in_degree(nid_t nid) const { return degree(nid); }
         nid_t out_degree(nid_t nid) const { return degree(nid); }
-        // Neighbours with a larger internal id, in insertion order (returned by
-        // reference, not copied as the reference does).
-        const std::vector<nid_t>& neighbors_with_geq_id(nid_t nid) const { return adjacency[nid]; }
+        // Neighbours with a larger internal id, in insertion order (a view, not the copy
+        // the reference returns).
+        NeighborRange neighbors_with_geq_id(nid_t nid) const
+        {
+            return NeighborRange{adj.data() + adj_off[nid], adj.data() + adj_off[nid + 1]};
+        }
 
         std::unordered_map<nid_t, nid_t> node_map;   // file id -> internal id
         std::vector<nid_t> node_map_r;               // internal id -> file id
@@ -49,9 +64,11 @@ namespace RPGraph
     private:
         nid_t node_count = 0, edge_count = 0;
         std::vector<nid_t> degrees;
-        std::vector<std::vector<nid_t>> adjacency;
+        std::vector<uint64_t> adj_off;   // CSR of each edge at its smaller internal id
+        std::vector<nid_t> adj;
         bool loaded = false;
-        void build(const std::vector<nid_t>& s, const std::vector<nid_t>& t);
+        void build(const nid_t* s, const nid_t* t, size_t count);
+        void finish(const std::vector<nid_t>& lo, const std::vector<nid_t>& hi);
         bool load_binary(const std::string& path);
     };
 }

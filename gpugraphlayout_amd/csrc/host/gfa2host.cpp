@@ -2,12 +2,15 @@
 #include "gfa2host.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "RPGraph.hpp"
 #include "RPGraphLayout.hpp"
+#include "hashing.hpp"
 
 struct gfa2host_graph
 {
@@ -133,4 +136,70 @@ extern "C" int64_t gfa2host_barabasi_albert(int32_t n, int32_t m, uint64_t seed,
         }
     }
     return e;
+}
+
+namespace
+{
+    // SplitMix64 output number k of a stream seeded with `seed` (counter form).
+    inline uint64_t splitmix_at(uint64_t seed, uint64_t k)
+    {
+        uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+}
+
+extern "C" int64_t gfa2host_rmat(int32_t scale, int64_t n_edges, double a, double b, double c, uint64_t seed,
+                                 int32_t* s, int32_t* t)
+{
+    if (scale < 1 || scale > 31 || n_edges < 0 || !(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0)) return -1;
+    const double pairs = std::ldexp(1.0, scale) * (std::ldexp(1.0, scale) - 1) / 2;
+    if ((double)n_edges > 0.5 * pairs) return -1;
+    const double ab = a + b, abc = a + b + c;
+    gfa2host::KeySet seen((uint64_t)n_edges);
+    const uint64_t max_draws = 64 * (uint64_t)n_edges + (1u << 20);   // ends skewed parameter sets
+    // Draws are computed in parallel a chunk at a time; the first-occurrence filter runs in
+    // draw order on one thread, so the result does not depend on the thread count.
+    constexpr uint64_t kChunk = 1u << 20;
+    std::vector<uint32_t> cu(kChunk), cv(kChunk);
+    unsigned nthreads = std::thread::hardware_concurrency();
+    nthreads = nthreads < 1 ? 1 : (nthreads > 16 ? 16 : nthreads);
+    int64_t got = 0;
+    for (uint64_t k0 = 0; got < n_edges && k0 < max_draws; k0 += kChunk)
+    {
+        auto work = [&](unsigned tid) {
+            const uint64_t lo = kChunk * tid / nthreads, hi = kChunk * (tid + 1) / nthreads;
+            for (uint64_t i = lo; i < hi; ++i)
+            {
+                const uint64_t k = k0 + i;
+                uint64_t u = 0, v = 0;
+                for (int l = 0; l < scale; ++l)
+                {
+                    const double x =
+                        (double)(splitmix_at(seed, k * (uint64_t)scale + (uint64_t)l) >> 11) * 0x1.0p-53;
+                    u = (u << 1) | (uint64_t)(x >= ab);
+                    v = (v << 1) | (uint64_t)(((x >= a) && (x < ab)) || (x >= abc));
+                }
+                cu[i] = (uint32_t)u;
+                cv[i] = (uint32_t)v;
+            }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned q = 1; q < nthreads; ++q) pool.emplace_back(work, q);
+        work(0);
+        for (auto& th : pool) th.join();
+        const uint64_t m = (max_draws - k0) < kChunk ? (max_draws - k0) : kChunk;
+        for (uint64_t i = 0; i < m && got < n_edges; ++i)
+        {
+            const uint64_t u = cu[i], v = cv[i];
+            if (u == v) continue;
+            const uint64_t lo = u < v ? u : v, hi = u < v ? v : u;
+            if (!seen.insert(((lo << scale) | hi) + 1)) continue;
+            s[got] = (int32_t)u;
+            t[got] = (int32_t)v;
+            ++got;
+        }
+    }
+    return got;
 }

@@ -111,31 +111,19 @@ def barabasi_albert(n: int, m: int, seed: int):
     return s[:got].astype(np.int64), t[:got].astype(np.int64)
 
 
-def rmat(scale: int, n_edges: int, a=0.57, b=0.19, c=0.19, seed: int = 1, batch: int = 1 << 21):
+def rmat(scale: int, n_edges: int, a=0.57, b=0.19, c=0.19, seed: int = 1):
     """R-MAT edges over 2^scale ids, drawn until n_edges distinct undirected non-loop
-    edges exist; returned in first-occurrence order with their drawn orientation."""
-    rng = np.random.default_rng(seed)
-    ab, abc = a + b, a + b + c
-    all_s, all_t, all_c = [], [], []
-    distinct = 0
-    while distinct < n_edges:
-        src = np.zeros(batch, np.int64)
-        dst = np.zeros(batch, np.int64)
-        for _ in range(scale):
-            x = rng.random(batch)
-            src = (src << 1) | (x >= ab)                            # quadrants c, d
-            dst = (dst << 1) | (((x >= a) & (x < ab)) | (x >= abc))  # quadrants b, d
-        ok = src != dst
-        src, dst = src[ok], dst[ok]
-        all_s.append(src)
-        all_t.append(dst)
-        all_c.append(np.minimum(src, dst) << scale | np.maximum(src, dst))
-        distinct = len(np.unique(np.concatenate(all_c)))
-    src, dst, code = np.concatenate(all_s), np.concatenate(all_t), np.concatenate(all_c)
-    _, first = np.unique(code, return_index=True)
-    first.sort()
-    first = first[:n_edges]
-    return src[first], dst[first]
+    edges exist; returned in first-occurrence order with their drawn orientation (the C++
+    generator of the host library: counter-based SplitMix64, one double per level)."""
+    from .host import lib as host_lib
+    s = np.zeros(n_edges, np.int32)
+    t = np.zeros(n_edges, np.int32)
+    got = host_lib().gfa2host_rmat(int(scale), int(n_edges), float(a), float(b), float(c),
+                                   int(seed) & (2 ** 64 - 1), s.ctypes.data_as(ctypes.c_void_p),
+                                   t.ctypes.data_as(ctypes.c_void_p))
+    if got < 0:
+        raise ValueError("invalid R-MAT arguments")
+    return s[:got].astype(np.int64), t[:got].astype(np.int64)
 
 
 def sbm(n: int, blocks: int, alpha: float, seed: int):

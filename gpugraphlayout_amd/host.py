@@ -28,6 +28,9 @@ def lib():
         L.gfa2host_free.restype = None
         L.gfa2host_barabasi_albert.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, _P, _P]
         L.gfa2host_barabasi_albert.restype = ctypes.c_int64
+        L.gfa2host_rmat.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint64, _P, _P]
+        L.gfa2host_rmat.restype = ctypes.c_int64
         _lib = L
     return _lib
 

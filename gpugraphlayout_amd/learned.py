@@ -59,7 +59,7 @@ def fa2_targets(gs, iters, chunk_nodes=4_000_000, seed0=0, log=None):
     return out
 
 
-def train(gs, targets, steps, lr=3e-3, R=8, seed=0, feature_steps=8, log=None):
+def train(gs, targets, steps, lr=3e-3, R=8, seed=0, feature_steps=2, log=None):
     m = Mpnn(seed=seed)
     for k, (g, t) in enumerate(zip(gs, targets)):
         rowptr, col = g.csr()
@@ -74,7 +74,7 @@ def train(gs, targets, steps, lr=3e-3, R=8, seed=0, feature_steps=8, log=None):
     return m
 
 
-def evaluate(m, g, seed, iters_plain=1000, check=10, feature_steps=8, nsources=1024, ntargets=977):
+def evaluate(m, g, seed, iters_plain=1000, check=10, feature_steps=2, nsources=1024, ntargets=977):
     rowptr, col = g.csr()
     # plain run
     e = Engine(g.n, g.src, g.dst, **FA2)

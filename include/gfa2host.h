@@ -38,6 +38,15 @@ void gfa2host_free(gfa2host_graph_t g);
  * Writes m*(n-m-1)+m edges (v, u) in creation order, u ascending per v. Returns the count. */
 int64_t gfa2host_barabasi_albert(int32_t n, int32_t m, uint64_t seed, int32_t* s, int32_t* t);
 
+/* R-MAT edges over 2^scale ids (quadrant probabilities a, b, c, 1-a-b-c), drawn until
+ * n_edges distinct undirected non-loop edges exist, in first-occurrence order with their
+ * drawn orientation. Draw k, level l uses SplitMix64 output number k*scale+l of the stream
+ * seeded with `seed`, as a double (z >> 11) * 2^-53; level 0 is the most significant id bit.
+ * Returns the edge count (fewer than n_edges only if 64*n_edges+2^20 draws did not reach
+ * it), or -1 for invalid arguments (scale outside 1..31, n_edges above half of all pairs). */
+int64_t gfa2host_rmat(int32_t scale, int64_t n_edges, double a, double b, double c, uint64_t seed, int32_t* s,
+                      int32_t* t);
+
 #ifdef __cplusplus
 }
 #endif

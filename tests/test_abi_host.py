@@ -71,6 +71,61 @@ def test_rmat_generator_invariants():
     assert deg.argmax() == 0
 
 
+def _splitmix_at(seed, k):
+    """SplitMix64 (Steele, Lea, Flood 2014) output number k of the stream seeded with seed."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (k.astype(np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _rmat_numpy(scale, n_edges, a, b, c, seed, batch=1 << 14):
+    """The R-MAT draw restated with numpy: all draws of a batch at once, first occurrences
+    of distinct undirected codes kept in draw order."""
+    ab, abc = a + b, a + b + c
+    ss, tt, cc = [], [], []
+    k0 = 0
+    while True:
+        k = np.arange(k0, k0 + batch, dtype=np.uint64)
+        u = np.zeros(batch, np.int64)
+        v = np.zeros(batch, np.int64)
+        for lvl in range(scale):
+            x = (_splitmix_at(seed, k * np.uint64(scale) + np.uint64(lvl)) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+            u = (u << 1) | (x >= ab)
+            v = (v << 1) | (((x >= a) & (x < ab)) | (x >= abc))
+        ok = u != v
+        ss.append(u[ok]); tt.append(v[ok]); cc.append(np.minimum(u[ok], v[ok]) << scale | np.maximum(u[ok], v[ok]))
+        code = np.concatenate(cc)
+        _, first = np.unique(code, return_index=True)
+        if len(first) >= n_edges:
+            first.sort()
+            first = first[:n_edges]
+            return np.concatenate(ss)[first], np.concatenate(tt)[first]
+        k0 += batch
+
+
+@pytest.mark.parametrize("scale,n_edges,abc,seed", [(12, 30000, (0.57, 0.19, 0.19), 1), (5, 200, (0.45, 0.15, 0.15), 9),
+                                                    (16, 100000, (0.25, 0.25, 0.25), 3)])
+def test_rmat_generator_matches_numpy_restatement(scale, n_edges, abc, seed):
+    s, t = graphs.rmat(scale, n_edges, *abc, seed=seed)
+    rs, rt = _rmat_numpy(scale, n_edges, *abc, seed)
+    np.testing.assert_array_equal(s, rs)
+    np.testing.assert_array_equal(t, rt)
+
+
+def test_rmat_generator_rejects_bad_arguments():
+    with pytest.raises(ValueError):
+        graphs.rmat(0, 10)
+    with pytest.raises(ValueError):
+        graphs.rmat(4, 100)      # more than half of the 120 pairs
+
+
+def test_splitmix_restatement_known_answer():
+    # SplitMix64 seeded with 0: the first output is the published 0xE220A8397B1DCDAF.
+    assert int(_splitmix_at(0, np.array([0], np.uint64))[0]) == 0xE220A8397B1DCDAF
+
+
 def test_sbm_generator_invariants():
     s, t, block = graphs.sbm(5000, 32, 2.0, 1000)
     g = graphs.edges_to_graph(s, t)

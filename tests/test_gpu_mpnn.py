@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from gpugraphlayout_amd import Engine, Mpnn, graphs
+from gpugraphlayout_amd import Engine, Mpnn, graphs, learned
 from oracle import mpnn_oracle as mo
 
 pytestmark = pytest.mark.gpu
@@ -92,32 +92,31 @@ def test_infer_to_engine_places_positions():
 
 
 def test_learned_start_beats_random_start():
-    """Trained on FA2 layouts of 12 seeded SBM graphs, the network's start has lower
-    normalized stress than the random start on held-out graphs. Whether FA2 from it then
-    reaches the plain run's final stress sooner is reported (printed), not asserted: on these
-    graphs plain FA2 plateaus within ~50 iterations and the two runs settle in basins whose
-    stress differs by a few 1e-4 either way."""
+    """Trained on FA2 layouts of 64 seeded SBM graphs, the network's start has lower
+    normalized stress than the random start on held-out graphs, on average and on most
+    graphs. Whether FA2 from it then reaches the plain run's final stress sooner is reported
+    (printed), not asserted: on these graphs plain FA2 plateaus within ~50 iterations and
+    runs settle in basins whose stress differs by a few 1e-4 either way."""
     print("case mpnn gain", flush=True)
-    n, it_plain = 1500, 300
-    m = Mpnn(seed=0)
-    for k in range(12):
+    n, it_plain, fsteps = 1500, 300, 2
+    train = []
+    for k in range(64):
         s, t, _ = graphs.sbm(n, 8, 2.0, 1000 + k)
-        g = graphs.edges_to_graph(s, t)
+        train.append(graphs.edges_to_graph(s, t))
+    targets = learned.fa2_targets(train, it_plain)
+    m = Mpnn(seed=0)
+    for k, (g, target) in enumerate(zip(train, targets)):
         rowptr, col = g.csr()
-        e = Engine(g.n, g.src, g.dst, scale=2.0)
-        e.set_positions(graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), k))
-        e.step(it_plain)
-        e.sync()
-        m.add_graph(g.n, rowptr, col, target=e.positions(), feature_seed=k)
-        e.close()
-    m.train(600, lr=3e-3, R=8, seed=2)
+        m.add_graph(g.n, rowptr, col, target=target, feature_seed=k, steps=fsteps)
+    m.train(1500, lr=3e-3, R=8, seed=2)
     scale = m.target_scale()
-    for k in range(3):
+    rand_ns, learned_ns = [], []
+    for k in range(8):
         s, t, _ = graphs.sbm(n, 8, 2.0, 5000 + k)
         g = graphs.edges_to_graph(s, t)
         rowptr, col = g.csr()
         rand_start = graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 100 + k)
-        plain = Engine(g.n, g.src, g.dst, scale=2.0)
+        plain = Engine(g.n, g.src, g.dst, **learned.FA2)
         plain.stress_setup(200, 100, seed=k)
         ns_random_start, _ = plain.normalized_stress(rand_start)
         plain.set_positions(rand_start)
@@ -125,8 +124,8 @@ def test_learned_start_beats_random_start():
         plain.sync()
         ns_plain, _ = plain.normalized_stress()
         plain.close()
-        gid = m.add_graph(g.n, rowptr, col, feature_seed=100 + k)
-        e = Engine(g.n, g.src, g.dst, scale=2.0)
+        gid = m.add_graph(g.n, rowptr, col, feature_seed=100 + k, steps=fsteps)
+        e = Engine(g.n, g.src, g.dst, **learned.FA2)
         m.infer_to_engine(gid, scale, e)
         e.stress_setup(200, 100, seed=k)
         ns_learned_start, _ = e.normalized_stress()
@@ -140,5 +139,8 @@ def test_learned_start_beats_random_start():
         e.close()
         print(f"held-out {k}: NS random start {ns_random_start:.4f}, learned start {ns_learned_start:.4f}; "
               f"plain after {it_plain} it {ns_plain:.4f}, learned run reaches it at {done}", flush=True)
-        assert ns_learned_start < ns_random_start
+        rand_ns.append(ns_random_start)
+        learned_ns.append(ns_learned_start)
     m.close()
+    assert np.mean(learned_ns) < np.mean(rand_ns) - 0.02
+    assert sum(a < b for a, b in zip(learned_ns, rand_ns)) >= 6
