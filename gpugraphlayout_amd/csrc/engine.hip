@@ -57,7 +57,7 @@ struct gfa2_engine {
     int groups_cap = 0;
     // launch shapes (autotuned per graph-size bucket)
     int cus = 256;
-    LaunchConfig tune{256, 8};
+    LaunchConfig tune{256, 8, 0};
 };
 
 namespace {
@@ -74,6 +74,7 @@ int size_bucket(int n) {
 
 void apply_tune(gfa2_engine* h) {
     h->p.walk_bt = h->tune.walk_bt;
+    h->p.walk_window = h->tune.walk_window;
     h->b.grid_cap = h->tune.blocks_per_cu * h->cus;
 }
 
@@ -730,6 +731,8 @@ int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     c->walk_block = h->tune.walk_bt;
     c->blocks_per_cu = h->tune.blocks_per_cu;
+    c->walk_window = h->tune.walk_window;
+    c->reserved = 0;
     c->ms_per_iteration = 0.0;
     return GFA2_OK;
 }
@@ -737,9 +740,10 @@ int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c) {
 int gfa2_set_launch_config(gfa2_handle h, const gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     const int bt = c->walk_block;
-    if ((bt != 64 && bt != 128 && bt != 256 && bt != 512) || c->blocks_per_cu < 1 || c->blocks_per_cu > 64)
+    if ((bt != 64 && bt != 128 && bt != 256 && bt != 512) || c->blocks_per_cu < 1 || c->blocks_per_cu > 64 ||
+        (c->walk_window != 0 && c->walk_window != 1))
         return GFA2_ERR_INVALID_ARG;
-    h->tune = LaunchConfig{bt, c->blocks_per_cu};
+    h->tune = LaunchConfig{bt, c->blocks_per_cu, c->walk_window};
     apply_tune(h);
     h->graph_valid = false;
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
@@ -754,6 +758,8 @@ int gfa2_tuned_config(int32_t nodes, gfa2_launch_config* c) {
     if (it == g_tune_table.end()) return GFA2_ERR_NO_GRAPH;
     c->walk_block = it->second.walk_bt;
     c->blocks_per_cu = it->second.blocks_per_cu;
+    c->walk_window = it->second.walk_window;
+    c->reserved = 0;
     c->ms_per_iteration = 0.0;
     return GFA2_OK;
 }
@@ -816,10 +822,11 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     float win_ms = 1e30f;
     const int bts[] = {64, 128, 256, 512};
     const int bpcs[] = {1, 2, 4, 8};
+    for (int wnd = 0; wnd < 2; ++wnd)
     for (int bt : bts) {
         for (int bpc : bpcs) {
             if (e != hipSuccess) break;
-            h->tune = LaunchConfig{bt, bpc};
+            h->tune = LaunchConfig{bt, bpc, wnd};
             apply_tune(h);
             enqueue_iteration(h, nullptr);   // warm
             e = hipEventRecord(t0, h->stream);
@@ -856,6 +863,8 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     if (best) {
         best->walk_block = win.walk_bt;
         best->blocks_per_cu = win.blocks_per_cu;
+        best->walk_window = win.walk_window;
+        best->reserved = 0;
         best->ms_per_iteration = win_ms / iters;
     }
     return GFA2_OK;

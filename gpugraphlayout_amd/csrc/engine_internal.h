@@ -81,12 +81,14 @@ struct Params {
     float itolsq, epssq;
     int group;              // walk vote group: 32 or 64 lanes
     int walk_bt = 256;      // walk threads per block (64..512); results do not depend on it
+    int walk_window = 0;    // records fetched a group-wide window at a time; results do not depend on it
 };
 
 // Launch shapes the autotuner chooses from; none of them changes any result bit.
 struct LaunchConfig {
     int walk_bt;            // 64, 128, 256, 512
     int blocks_per_cu;      // grid of the grid-stride kernels (keys, displacement, bbox)
+    int walk_window;        // 0: one record load per step; 1: group-wide record windows
 };
 
 // All device buffers of one instance.

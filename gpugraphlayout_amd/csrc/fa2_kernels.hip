@@ -119,7 +119,11 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
 // The epilogue adds gravity + attraction, writes the total force, and forms the group's
 // swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
 // ---------------------------------------------------------------------------------------
-template <int G, int BT>
+// WIN: the group's lanes load G consecutive records at once (one float4 each, coalesced)
+// and the walk reads record r from lane r - window_base by a lane shuffle while r stays in
+// that window; only a step that leaves it waits on memory. Same records, same order, same
+// arithmetic: the results do not depend on WIN.
+template <int G, int BT, bool WIN>
 __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, const int* __restrict__ brec,
                                               const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
                                               int ngroups, const int* __restrict__ node_off,
@@ -167,8 +171,24 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         const float base = two_r * two_r * itolsq;
         int r = rec_begin[g] + 1;
         const int end = rec_begin[g + 1];
+        int wb = r - G;                       // window [wb, wb + G): empty at the start
+        float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         while (r < end) {
-            const float4 c = rec[r];
+            float4 c;
+            if constexpr (WIN) {
+                if (r - wb >= G) {            // uniform within the group
+                    wb = r;
+                    const int q = r + gl;
+                    w = q < end ? rec[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                const int src = gbase + (r - wb);
+                c.x = __shfl(w.x, src, 64);
+                c.y = __shfl(w.y, src, 64);
+                c.z = __shfl(w.z, src, 64);
+                c.w = __shfl(w.w, src, 64);
+            } else {
+                c = rec[r];
+            }
             const unsigned int info = __float_as_uint(c.w);
             const float dx = px - c.x;
             const float dy = py - c.y;
@@ -369,9 +389,14 @@ namespace {
 
 template <int G, int BT>
 void walk_launch(const Bufs& b, const Params& p, hipStream_t s) {
-    k_walk<G, BT><<<walk_blocks(b.ngroups, G, BT), BT, 0, s>>>(
-        b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev, b.f,
-        b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
+    if (p.walk_window)
+        k_walk<G, BT, true><<<walk_blocks(b.ngroups, G, BT), BT, 0, s>>>(
+            b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev,
+            b.f, b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
+    else
+        k_walk<G, BT, false><<<walk_blocks(b.ngroups, G, BT), BT, 0, s>>>(
+            b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev,
+            b.f, b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
 }
 
 template <int G>

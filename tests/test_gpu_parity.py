@@ -206,16 +206,20 @@ def test_coincident_bodies_are_finite():
     eng.close()
 
 
-@pytest.mark.parametrize("walk_block,blocks_per_cu", [(64, 1), (128, 2), (512, 8), (256, 3)])
-def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu):
-    print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu}", flush=True)
+@pytest.mark.parametrize("walk_block,blocks_per_cu,window,group", [
+    (64, 1, 0, 32), (128, 2, 0, 32), (512, 8, 0, 32), (256, 3, 0, 32),
+    (64, 8, 1, 32), (256, 4, 1, 32), (512, 1, 1, 32), (128, 8, 1, 64), (256, 2, 0, 64)])
+def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, window, group):
+    print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu} window={window} "
+          f"group={group}", flush=True)
     g, pos = make_case("rmat", 43)
     res = []
-    for shape in (None, (walk_block, blocks_per_cu)):
-        eng = Engine(g.n, g.src, g.dst, scale=10.0)
+    for shape in (None, (walk_block, blocks_per_cu, window)):
+        eng = Engine(g.n, g.src, g.dst, scale=10.0, group_size=group)
         if shape is not None:
             eng.set_launch_config(*shape)
             assert eng.launch_config()["walk_block"] == walk_block
+            assert eng.launch_config()["walk_window"] == window
         eng.set_positions(pos)
         eng.step(20)
         eng.sync()
