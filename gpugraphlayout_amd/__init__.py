@@ -81,7 +81,7 @@ class LaunchConfig(ctypes.Structure):
     _fields_ = [
         ("walk_block", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_int32),
-        ("walk_window", ctypes.c_int32),
+        ("walk_fetch", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("ms_per_iteration", ctypes.c_double),
     ]
@@ -308,8 +308,8 @@ class Engine:
         _check(lib().gfa2_get_launch_config(self.h, ctypes.byref(c)), "gfa2_get_launch_config")
         return c.as_dict()
 
-    def set_launch_config(self, walk_block: int, blocks_per_cu: int, walk_window: int = 0):
-        c = LaunchConfig(int(walk_block), int(blocks_per_cu), int(walk_window), 0, 0.0)
+    def set_launch_config(self, walk_block: int, blocks_per_cu: int, walk_fetch: int = 0):
+        c = LaunchConfig(int(walk_block), int(blocks_per_cu), int(walk_fetch), 0, 0.0)
         _check(lib().gfa2_set_launch_config(self.h, ctypes.byref(c)), "gfa2_set_launch_config")
 
     def set_timing(self, on: bool = True):

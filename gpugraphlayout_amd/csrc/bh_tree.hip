@@ -210,19 +210,18 @@ __device__ Mass3 block_exclusive_scan3(Mass3 v, Mass3* total) {
     return Mass3{before.m + a - v.m, before.x + b - v.x, before.y + c - v.y};
 }
 
-// Mass and mass-weighted position (relative to the graph's root centre) of sorted body i.
-__device__ __forceinline__ Mass3 body_m3(int i, const unsigned int* __restrict__ vals,
-                                         const float2* __restrict__ pos, const float* __restrict__ mass,
-                                         const int* __restrict__ sg, const float4* __restrict__ seg_root) {
-    const unsigned int id = vals[i];
-    const float2 p = pos[id];
-    const float4 c = seg_root[sg ? sg[i] : 0];
-    const double m = mass[id];
-    return Mass3{m, m * ((double)p.x - c.x), m * ((double)p.y - c.y)};
+// Mass and mass-weighted position (relative to the graph's root centre) of a sorted body
+// {x, y, mass} of graph g.
+__device__ __forceinline__ Mass3 body_m3(const float4 bd, int g, const float4* __restrict__ seg_root) {
+    const float4 c = seg_root[g];
+    const double m = bd.z;
+    return Mass3{m, m * ((double)bd.x - c.x), m * ((double)bd.y - c.y)};
 }
 
 // lcp per sorted position and the number of cells opened there; block sums (of the
-// cell counts and, on the integer-mass path, of body mass sums).
+// cell counts and, on the integer-mass path, of body mass sums). Each sorted body's
+// {x, y, mass} is gathered here once and written in sorted order (sbody) for the kernels
+// that follow.
 __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* __restrict__ keys,
                                                       const int* __restrict__ sg, int n,
                                                       signed char* __restrict__ lcp,
@@ -231,7 +230,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
                                                       const float2* __restrict__ pos,
                                                       const float* __restrict__ mass,
                                                       const float4* __restrict__ seg_root,
-                                                      Mass3* __restrict__ bmass) {
+                                                      Mass3* __restrict__ bmass, float4* __restrict__ sbody) {
     const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
     int sum = 0;
     Mass3 ms{0.0, 0.0, 0.0};
@@ -243,7 +242,11 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
             int l = lcp_at(keys, sg, n, i);
             lcp[i] = (signed char)l;
             sum += max(0, l - prev);
-            if (bmass) ms = m3_add(ms, body_m3(i, vals, pos, mass, sg, seg_root));
+            const unsigned int id = vals[i];
+            const float2 p = pos[id];
+            const float4 bd = make_float4(p.x, p.y, mass[id], 0.0f);
+            sbody[i] = bd;
+            if (bmass) ms = m3_add(ms, body_m3(bd, sg ? sg[i] : 0, seg_root));
         }
     }
     int total;
@@ -321,20 +324,21 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
                                                          const int* __restrict__ sg, int n,
                                                          const int* __restrict__ blockoff,
                                                          int* __restrict__ S,
-                                                         const unsigned int* __restrict__ vals,
-                                                         const float2* __restrict__ pos,
-                                                         const float* __restrict__ mass,
+                                                         const float4* __restrict__ sbody,
                                                          float4* __restrict__ rec, int* __restrict__ brec,
                                                          const DevState* __restrict__ st,
                                                          const float4* __restrict__ seg_root,
-                                                         const Mass3* __restrict__ bmass, Mass3* __restrict__ pmass) {
+                                                         const Mass3* __restrict__ bmass, Mass3* __restrict__ pmass,
+                                                         int* __restrict__ cell_pos, int max_cells) {
     const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
     int s[kTreeItems];
+    float4 bd[kTreeItems];
     int sum = 0;
 #pragma unroll
     for (int k = 0; k < kTreeItems; ++k) {
         int i = i0 + k;
         s[k] = i < n ? max(0, (int)lcp[i] - lcp_prev(lcp, sg, i)) : 0;
+        bd[k] = i < n ? sbody[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         sum += s[k];
     }
     int total;
@@ -343,15 +347,17 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
     if (pmass) {
         // exclusive prefix of body mass sums over sorted positions
         Mass3 mine{0.0, 0.0, 0.0};
+#pragma unroll
         for (int k = 0; k < kTreeItems; ++k)
-            if (i0 + k < n) mine = m3_add(mine, body_m3(i0 + k, vals, pos, mass, sg, seg_root));
+            if (i0 + k < n) mine = m3_add(mine, body_m3(bd[k], sg ? sg[i0 + k] : 0, seg_root));
         Mass3 tot;
         Mass3 pre = m3_add(block_exclusive_scan3<kTreeThreads>(mine, &tot), bmass[blockIdx.x]);
+#pragma unroll
         for (int k = 0; k < kTreeItems; ++k) {
             const int i = i0 + k;
             if (i < n) {
                 pmass[i] = pre;
-                pre = m3_add(pre, body_m3(i, vals, pos, mass, sg, seg_root));
+                pre = m3_add(pre, body_m3(bd[k], sg ? sg[i] : 0, seg_root));
                 if (i == n - 1) pmass[n] = pre;
             }
         }
@@ -361,20 +367,24 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
         int i = i0 + k;
         if (i < n) {
             S[i] = run;
+            if (ok)
+                for (int j = 0; j < s[k] && run + j < max_cells; ++j) cell_pos[run + j] = i;
             run += s[k];
             int rb = i + run;          // after this position's own cells
             brec[i] = rb;
-            if (ok) {
-                unsigned int id = vals[i];
-                float2 p = pos[id];
-                rec[rb] = make_float4(p.x, p.y, mass[id], __int_as_float(-1));
-            }
+            if (ok) rec[rb] = make_float4(bd[k].x, bd[k].y, bd[k].z, __int_as_float(-1));
         }
     }
     if (i0 + kTreeItems >= n && i0 < n) S[n] = run;   // total cells
 }
 
-// Cell records (skip, level, count) and the cell list for level bucketing.
+// Cell records (skip, level, count) and the cell list for level bucketing, one thread per
+// cell (grid-stride over the S[n] cells). cell_rec[ord] holds the cell's first sorted
+// position on entry (written by k_prefix) and its record index on exit. The cells opened at
+// position i have levels lprev+1 .. lcp_i in ord order; each finds its extent end by a
+// galloping search over the keys (sorted within a graph, so a cell's bodies are contiguous).
+// One thread per cell keeps the long searches of the top levels, which all start at a
+// graph's first position, in parallel.
 __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long* __restrict__ keys,
                                                         const signed char* __restrict__ lcp,
                                                         const int* __restrict__ sg, int n,
@@ -390,66 +400,60 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
     if (threadIdx.x == 0) lmax = 0;
     __syncthreads();
     const bool ok = st->err == 0;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ok && i < n) {
-        const int lprev = lcp_prev(lcp, sg, i);
-        const int lcur = (int)lcp[i];
-        if (lcur > lprev) {
-            const unsigned long long ki = keys[i];
-            const int gi = sg ? sg[i] : 0;
-            const int Si = S[i];
-            int lo = i;     // known member of the current cell's extent
-            for (int L = lcur; L > lprev; --L) {
-                // extent end: largest j of the same graph with lcp(k_i, k_j) >= L
-                // (keys sorted within a graph => contiguous)
-                int step = 1;
-                int hi = lo + 1;
-                while (hi < n && (!sg || sg[hi] == gi) && lcp_levels(ki, keys[hi]) >= L) {
-                    lo = hi;
-                    step <<= 1;
-                    hi = lo + step;
-                }
-                if (hi > n) hi = n;
-                while (hi - lo > 1) {
-                    int mid = lo + ((hi - lo) >> 1);
-                    if ((!sg || sg[mid] == gi) && lcp_levels(ki, keys[mid]) >= L) lo = mid;
-                    else hi = mid;
-                }
-                const int b = lo;
-                const int ord = Si + (L - lprev - 1);
-                const int rc = i + ord;
-                const unsigned int skip = (unsigned int)(b + S[b + 1] + 1 - rc);
-                const unsigned int info = (skip << kInfoLevelBits) | (unsigned int)((L > 0 ? L - 1 : 0) & 31);
-                if (pmass) {
-                    // whole-number masses: a cell lighter than 2^24 has this exact float mass
-                    // in any summation order; heavier cells are summed in quadrant order later
-                    const Mass3 a = pmass[i], z = pmass[b + 1];
-                    const double M = z.m - a.m;
-                    if (M < 16777216.0) {
-                        const float4 c = seg_root[gi];
-                        rec[rc] = make_float4((float)((z.x - a.x) / M + c.x), (float)((z.y - a.y) / M + c.y), (float)M,
-                                              __uint_as_float(info));
-                    } else {
-                        rec[rc].w = __uint_as_float(info);
-                        const int slot = atomicAdd(&st->nbig, 1);
-                        if (slot < kMaxBig) {
-                            big_rc[slot] = rc;
-                            big_lvl[slot] = (unsigned char)L;
-                        } else {
-                            st->err |= kErrTreeOverflow;
-                        }
-                    }
-                } else {
-                    rec[rc].w = __uint_as_float(info);
-                }
-                rcount[rc] = b - i + 1;
-                cell_rec[ord] = rc;
-                cell_lvl[ord] = (unsigned char)L;
-                if (L == 0 && rec_begin) rec_begin[gi] = rc;
-                atomicAdd(&hist[L], 1);
-            }
-            atomicMax(&lmax, lcur);
+    const int ncells = ok ? S[n] : 0;
+    for (int ord = blockIdx.x * blockDim.x + threadIdx.x; ord < ncells; ord += gridDim.x * blockDim.x) {
+        const int i = cell_rec[ord];
+        const int Si = S[i];
+        const int L = lcp_prev(lcp, sg, i) + 1 + (ord - Si);
+        const unsigned long long ki = keys[i];
+        const int gi = sg ? sg[i] : 0;
+        // extent end: largest j of the same graph with lcp(k_i, k_j) >= L
+        int lo = i;
+        int step = 1;
+        int hi = lo + 1;
+        while (hi < n && (!sg || sg[hi] == gi) && lcp_levels(ki, keys[hi]) >= L) {
+            lo = hi;
+            step <<= 1;
+            hi = lo + step;
         }
+        if (hi > n) hi = n;
+        while (hi - lo > 1) {
+            int mid = lo + ((hi - lo) >> 1);
+            if ((!sg || sg[mid] == gi) && lcp_levels(ki, keys[mid]) >= L) lo = mid;
+            else hi = mid;
+        }
+        const int b = lo;
+        const int rc = i + ord;
+        const unsigned int skip = (unsigned int)(b + S[b + 1] + 1 - rc);
+        const unsigned int info = (skip << kInfoLevelBits) | (unsigned int)((L > 0 ? L - 1 : 0) & 31);
+        if (pmass) {
+            // whole-number masses: a cell lighter than 2^24 has this exact float mass in any
+            // summation order; heavier cells are summed in quadrant order later
+            const Mass3 a = pmass[i], z = pmass[b + 1];
+            const double M = z.m - a.m;
+            if (M < 16777216.0) {
+                const float4 c = seg_root[gi];
+                rec[rc] = make_float4((float)((z.x - a.x) / M + c.x), (float)((z.y - a.y) / M + c.y), (float)M,
+                                      __uint_as_float(info));
+            } else {
+                rec[rc].w = __uint_as_float(info);
+                const int slot = atomicAdd(&st->nbig, 1);
+                if (slot < kMaxBig) {
+                    big_rc[slot] = rc;
+                    big_lvl[slot] = (unsigned char)L;
+                } else {
+                    st->err |= kErrTreeOverflow;
+                }
+            }
+        } else {
+            rec[rc].w = __uint_as_float(info);
+        }
+        rcount[rc] = b - i + 1;
+        cell_rec[ord] = rc;
+        cell_lvl[ord] = (unsigned char)L;
+        if (L == 0 && rec_begin) rec_begin[gi] = rc;
+        atomicAdd(&hist[L], 1);
+        atomicMax(&lmax, L);
     }
     __syncthreads();
     if (threadIdx.x <= kMaxLevels && hist[threadIdx.x] != 0) atomicAdd(&st->level_count[threadIdx.x], hist[threadIdx.x]);
@@ -624,19 +628,21 @@ void launch_tree(Bufs& b, hipStream_t s) {
     Mass3* bm = b.int_mass ? b.bmass : nullptr;
     Mass3* pm = b.int_mass ? b.pmass : nullptr;
     k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum, b.sort.vals[0], b.pos, b.mass,
-                                         b.seg_root, bm);
+                                         b.seg_root, bm, b.sbody);
     k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st, bm);
-    k_prefix<<<tiles, kTreeThreads, 0, s>>>(b.lcp, sg, n, b.blocksum, b.S, b.sort.vals[0], b.pos, b.mass, b.rec,
-                                            b.brec, b.st, b.seg_root, bm, pm);
-    k_nodes<<<(n + kTreeThreads - 1) / kTreeThreads, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, sg, n, b.S, b.rec,
-                                                                           b.rcount, b.cell_rec, b.cell_lvl,
-                                                                           b.rec_begin, b.st, pm, b.seg_root,
-                                                                           b.big_rc, b.big_lvl);
+    const int max_cells = b.cap_records - n;
+    k_prefix<<<tiles, kTreeThreads, 0, s>>>(b.lcp, sg, n, b.blocksum, b.S, b.sbody, b.rec, b.brec, b.st, b.seg_root,
+                                            bm, pm, b.cell_rec, max_cells);
+    // grid-stride over the cells (their number is known on the device only; at most
+    // max_cells, typically below n)
+    const int node_blocks = (n + kTreeThreads - 1) / kTreeThreads;
+    k_nodes<<<node_blocks, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, sg, n, b.S, b.rec, b.rcount, b.cell_rec,
+                                                 b.cell_lvl, b.rec_begin, b.st, pm, b.seg_root, b.big_rc,
+                                                 b.big_lvl);
     if (b.int_mass) {
         k_big<<<1, 256, 0, s>>>(b.rec, b.big_rc, b.big_lvl, b.st);
         return;
     }
-    const int max_cells = b.cap_records - n;
     k_levels<<<(max_cells + 255) / 256, 256, 0, s>>>(b.cell_rec, b.cell_lvl, b.level_list, b.st);
     int sum_blocks = (max_cells / 4 + 255) / 256;
     if (sum_blocks < 1) sum_blocks = 1;

@@ -33,6 +33,7 @@ struct gfa2_engine {
     bool has_pos = false;
     long long edges = 0;
     bool capture_rep = false;
+    float2* rep_buf = nullptr;          // repulsion of the last captured iteration (kept when capture stops)
     std::vector<void*> allocs;
     // timing
     bool timing = false;
@@ -74,7 +75,7 @@ int size_bucket(int n) {
 
 void apply_tune(gfa2_engine* h) {
     h->p.walk_bt = h->tune.walk_bt;
-    h->p.walk_window = h->tune.walk_window;
+    h->p.walk_fetch = h->tune.walk_fetch;
     h->b.grid_cap = h->tune.blocks_per_cu * h->cus;
 }
 
@@ -112,6 +113,7 @@ void free_all(gfa2_engine* h) {
     for (void* p : h->allocs) hipFree(p);
     h->allocs.clear();
     h->b = Bufs{};
+    h->rep_buf = nullptr;
     h->has_graph = false;
     h->has_pos = false;
     h->stress_K = h->stress_T = 0;
@@ -407,6 +409,7 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.S, (size_t)n + 1);
     GFA2_ALLOC(b.rec, (size_t)cap);
     GFA2_ALLOC(b.brec, n);
+    GFA2_ALLOC(b.sbody, n);
     GFA2_ALLOC(b.rcount, (size_t)cap);
     GFA2_ALLOC(b.cell_rec, maxcells);
     GFA2_ALLOC(b.cell_lvl, maxcells);
@@ -428,7 +431,11 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
         GFA2_ALLOC(b.pkey, n);
         GFA2_TRY(hipMemcpyAsync(so, seg_of.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
     }
-    if (h->capture_rep) GFA2_ALLOC(b.rep, n);
+    h->rep_buf = nullptr;
+    if (h->capture_rep) {
+        GFA2_ALLOC(h->rep_buf, n);
+        b.rep = h->rep_buf;
+    }
     {
         double total = 0.0;
         bool whole = true;
@@ -608,7 +615,7 @@ int gfa2_get_forces(gfa2_handle h, int32_t kind, float* xy) {
     GFA2_TRY(hipSetDevice(h->device));
     const float2* src = nullptr;
     if (kind == GFA2_FORCE_TOTAL) src = h->b.f;
-    else if (kind == GFA2_FORCE_REPULSION) src = h->b.rep;
+    else if (kind == GFA2_FORCE_REPULSION) src = h->rep_buf;
     else if (kind == GFA2_FORCE_ATTRACTION_GRAVITY) src = h->b.fga;
     else return GFA2_ERR_INVALID_ARG;
     if (!src) return GFA2_ERR_INVALID_ARG;
@@ -620,13 +627,14 @@ int gfa2_get_forces(gfa2_handle h, int32_t kind, float* xy) {
 int gfa2_set_capture(gfa2_handle h, int32_t on) {
     if (!h) return GFA2_ERR_INVALID_ARG;
     h->capture_rep = on != 0;
-    if (h->has_graph && h->capture_rep && !h->b.rep) {
+    if (h->has_graph && h->capture_rep && !h->rep_buf) {
         GFA2_TRY(hipSetDevice(h->device));
-        int st = dalloc(h, &h->b.rep, h->b.n);
+        int st = dalloc(h, &h->rep_buf, h->b.n);
         if (st != GFA2_OK) return st;
-        GFA2_TRY(hipMemsetAsync(h->b.rep, 0, sizeof(float2) * h->b.n, h->stream));
+        GFA2_TRY(hipMemsetAsync(h->rep_buf, 0, sizeof(float2) * h->b.n, h->stream));
     }
-    if (h->has_graph && !h->capture_rep) h->b.rep = nullptr;
+    // the walk writes the repulsion only while capture is on; the last capture stays readable
+    if (h->has_graph) h->b.rep = h->capture_rep ? h->rep_buf : nullptr;
     h->graph_valid = false;
     if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
@@ -731,7 +739,7 @@ int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     c->walk_block = h->tune.walk_bt;
     c->blocks_per_cu = h->tune.blocks_per_cu;
-    c->walk_window = h->tune.walk_window;
+    c->walk_fetch = h->tune.walk_fetch;
     c->reserved = 0;
     c->ms_per_iteration = 0.0;
     return GFA2_OK;
@@ -741,9 +749,9 @@ int gfa2_set_launch_config(gfa2_handle h, const gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     const int bt = c->walk_block;
     if ((bt != 64 && bt != 128 && bt != 256 && bt != 512) || c->blocks_per_cu < 1 || c->blocks_per_cu > 64 ||
-        (c->walk_window != 0 && c->walk_window != 1))
+        c->walk_fetch < 0 || c->walk_fetch > 2)
         return GFA2_ERR_INVALID_ARG;
-    h->tune = LaunchConfig{bt, c->blocks_per_cu, c->walk_window};
+    h->tune = LaunchConfig{bt, c->blocks_per_cu, c->walk_fetch};
     apply_tune(h);
     h->graph_valid = false;
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
@@ -758,7 +766,7 @@ int gfa2_tuned_config(int32_t nodes, gfa2_launch_config* c) {
     if (it == g_tune_table.end()) return GFA2_ERR_NO_GRAPH;
     c->walk_block = it->second.walk_bt;
     c->blocks_per_cu = it->second.blocks_per_cu;
-    c->walk_window = it->second.walk_window;
+    c->walk_fetch = it->second.walk_fetch;
     c->reserved = 0;
     c->ms_per_iteration = 0.0;
     return GFA2_OK;
@@ -820,28 +828,34 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     const LaunchConfig keep = h->tune;
     LaunchConfig win = keep;
     float win_ms = 1e30f;
-    const int bts[] = {64, 128, 256, 512};
-    const int bpcs[] = {1, 2, 4, 8};
-    for (int wnd = 0; wnd < 2; ++wnd)
-    for (int bt : bts) {
-        for (int bpc : bpcs) {
-            if (e != hipSuccess) break;
-            h->tune = LaunchConfig{bt, bpc, wnd};
-            apply_tune(h);
-            enqueue_iteration(h, nullptr);   // warm
-            e = hipEventRecord(t0, h->stream);
-            for (int i = 0; i < iters && e == hipSuccess; ++i) enqueue_iteration(h, nullptr);
-            if (e == hipSuccess) e = hipEventRecord(t1, h->stream);
-            if (e == hipSuccess) e = hipEventSynchronize(t1);
-            float ms = 0.0f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
-            if (e == hipSuccess && ms < win_ms) {
-                win_ms = ms;
-                win = h->tune;
-            }
-            if (e == hipSuccess) e = copy_all(false);
+    // Times one candidate over `iters` iterations after one untimed one, then restores the state.
+    auto trial = [&](LaunchConfig cand) {
+        if (e != hipSuccess) return;
+        h->tune = cand;
+        apply_tune(h);
+        enqueue_iteration(h, nullptr);   // warm
+        e = hipEventRecord(t0, h->stream);
+        for (int i = 0; i < iters && e == hipSuccess; ++i) enqueue_iteration(h, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(t1, h->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e == hipSuccess && ms < win_ms) {
+            win_ms = ms;
+            win = cand;
         }
-    }
+        if (e == hipSuccess) e = copy_all(false);
+    };
+    // Stage 1: the walk's block size and record window (the walk dominates); stage 2: the
+    // grid of the grid-stride kernels, with the stage-1 winner.
+    const int bts[] = {64, 128, 256, 512};
+    const int fetches[] = {0, 1, 2};
+    const int bpcs[] = {1, 2, 4, 8};
+    for (int fetch : fetches)
+        for (int bt : bts) trial(LaunchConfig{bt, keep.blocks_per_cu, fetch});
+    const LaunchConfig shape = win;
+    for (int bpc : bpcs)
+        if (bpc != shape.blocks_per_cu) trial(LaunchConfig{shape.walk_bt, bpc, shape.walk_fetch});
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     for (void* p : {(void*)spos, (void*)sfp, (void*)sf, (void*)sfga, (void*)sst, (void*)sseg, (void*)spart})
@@ -863,7 +877,7 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     if (best) {
         best->walk_block = win.walk_bt;
         best->blocks_per_cu = win.blocks_per_cu;
-        best->walk_window = win.walk_window;
+        best->walk_fetch = win.walk_fetch;
         best->reserved = 0;
         best->ms_per_iteration = win_ms / iters;
     }

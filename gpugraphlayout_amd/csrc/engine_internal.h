@@ -81,14 +81,14 @@ struct Params {
     float itolsq, epssq;
     int group;              // walk vote group: 32 or 64 lanes
     int walk_bt = 256;      // walk threads per block (64..512); results do not depend on it
-    int walk_window = 0;    // records fetched a group-wide window at a time; results do not depend on it
+    int walk_fetch = 0;     // walk record fetch mode (0 direct, 1 window, 2 paired); no result depends on it
 };
 
 // Launch shapes the autotuner chooses from; none of them changes any result bit.
 struct LaunchConfig {
     int walk_bt;            // 64, 128, 256, 512
     int blocks_per_cu;      // grid of the grid-stride kernels (keys, displacement, bbox)
-    int walk_window;        // 0: one record load per step; 1: group-wide record windows
+    int walk_fetch;         // 0: one record load per step; 1: group-wide windows; 2: records in pairs
 };
 
 // All device buffers of one instance.
@@ -113,6 +113,7 @@ struct Bufs {
     int* S;
     float4* rec;
     int* brec;
+    float4* sbody;          // {x, y, mass, 0} of the body at sorted position k
     int* rcount;
     int* cell_rec;
     unsigned char* cell_lvl;

@@ -119,11 +119,43 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
 // The epilogue adds gravity + attraction, writes the total force, and forms the group's
 // swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
 // ---------------------------------------------------------------------------------------
-// WIN: the group's lanes load G consecutive records at once (one float4 each, coalesced)
-// and the walk reads record r from lane r - window_base by a lane shuffle while r stays in
-// that window; only a step that leaves it waits on memory. Same records, same order, same
-// arithmetic: the results do not depend on WIN.
-template <int G, int BT, bool WIN>
+// One walk step on record c for the group (G lanes): the vote, the accumulation and the
+// advance of the record index (1, or the subtree size of an accepted cell). A step that is
+// not live contributes nothing: fma(d, 0, a) == a, since a is never -0 here.
+template <int G>
+__device__ __forceinline__ int walk_step(const float4 c, bool live, bool active, float px, float py, float kmi,
+                                         float base, float epssq, float& ax, float& ay, unsigned int& inter) {
+    const unsigned int info = __float_as_uint(c.w);
+    const float dx = px - c.x;
+    const float dy = py - c.y;
+    const float d2 = dx * dx + dy * dy + epssq;
+    const bool body = info == 0xffffffffu;
+    const float thr = ldexpf(base, -2 * (int)(info & 31u)) + epssq;
+    const unsigned long long fails = __builtin_amdgcn_ballot_w64(live && active && !body && !(d2 >= thr));
+    // The group's vote as a lane mask built with scalar operations (each half-wave is one
+    // group when G == 32), read back per lane without touching vector registers.
+    unsigned long long okm;
+    if constexpr (G == 32)
+        okm = ((unsigned int)fails == 0u ? 0xffffffffull : 0ull) |
+              ((unsigned int)(fails >> 32) == 0u ? 0xffffffff00000000ull : 0ull);
+    else
+        okm = fails == 0ull ? ~0ull : 0ull;
+    const bool pass = __builtin_amdgcn_inverse_ballot_w64(okm);
+    const bool accept = live & (body | pass);   // no short circuit: keeps the step branch-free
+    const float w = accept ? kmi * c.z * __builtin_amdgcn_rcpf(d2) : 0.0f;
+    ax = fmaf(dx, w, ax);
+    ay = fmaf(dy, w, ay);
+    inter += (active && accept) ? 1u : 0u;
+    return (accept && !body) ? max((int)(info >> kInfoLevelBits), 1) : 1;
+}
+
+// Record fetch modes (none changes a result bit: same records, same order, same arithmetic):
+//   0  one load of record r per step;
+//   1  the group's lanes load G consecutive records at once (coalesced) and a step reads
+//      record r from the lane holding it by a lane shuffle while r stays in that window;
+//   2  records r and r+1 are loaded together and, when the step on r moves to r+1, the
+//      step on r+1 follows without another wait on memory.
+template <int G, int BT, int MODE>
 __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, const int* __restrict__ brec,
                                               const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
                                               int ngroups, const int* __restrict__ node_off,
@@ -171,42 +203,35 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         const float base = two_r * two_r * itolsq;
         int r = rec_begin[g] + 1;
         const int end = rec_begin[g + 1];
-        int wb = r - G;                       // window [wb, wb + G): empty at the start
-        float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        while (r < end) {
-            float4 c;
-            if constexpr (WIN) {
+        if constexpr (MODE == 1) {
+            int wb = r - G;                   // window [wb, wb + G): empty at the start
+            float4 wnd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            while (r < end) {
                 if (r - wb >= G) {            // uniform within the group
                     wb = r;
                     const int q = r + gl;
-                    w = q < end ? rec[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    wnd = q < end ? rec[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
                 const int src = gbase + (r - wb);
-                c.x = __shfl(w.x, src, 64);
-                c.y = __shfl(w.y, src, 64);
-                c.z = __shfl(w.z, src, 64);
-                c.w = __shfl(w.w, src, 64);
-            } else {
-                c = rec[r];
+                float4 c;
+                c.x = __shfl(wnd.x, src, 64);
+                c.y = __shfl(wnd.y, src, 64);
+                c.z = __shfl(wnd.z, src, 64);
+                c.w = __shfl(wnd.w, src, 64);
+                r += walk_step<G>(c, true, active, px, py, kmi, base, epssq, ax, ay, inter);
             }
-            const unsigned int info = __float_as_uint(c.w);
-            const float dx = px - c.x;
-            const float dy = py - c.y;
-            const float d2 = dx * dx + dy * dy + epssq;
-            const bool body = info == 0xffffffffu;
-            const float thr = ldexpf(base, -2 * (int)(info & 31u)) + epssq;
-            const unsigned long long fails = __ballot(active && !body && !(d2 >= thr));
-            const unsigned int gf = (G == 32) ? (unsigned int)(fails >> gbase) : (fails != 0ull ? 1u : 0u);
-            const bool accept = body || gf == 0u;
-            if (accept) {
-                const float w = kmi * c.z * __builtin_amdgcn_rcpf(d2);
-                ax = fmaf(dx, w, ax);
-                ay = fmaf(dy, w, ay);
-                inter += active ? 1u : 0u;
-                r += body ? 1 : max((int)(info >> kInfoLevelBits), 1);
-            } else {
-                r += 1;
+        } else if constexpr (MODE == 2) {
+            while (r < end) {
+                const bool has1 = r + 1 < end;
+                const float4 c0 = rec[r];
+                const float4 c1 = rec[has1 ? r + 1 : r];
+                const int a0 = walk_step<G>(c0, true, active, px, py, kmi, base, epssq, ax, ay, inter);
+                const bool live1 = has1 && a0 == 1;
+                const int a1 = walk_step<G>(c1, live1, active, px, py, kmi, base, epssq, ax, ay, inter);
+                r += live1 ? 1 + a1 : a0;
             }
+        } else {
+            while (r < end) r += walk_step<G>(rec[r], true, active, px, py, kmi, base, epssq, ax, ay, inter);
         }
     }
     float sw = 0.0f, tr = 0.0f;
@@ -387,16 +412,20 @@ int walk_blocks(int ngroups, int group, int bt) {
 
 namespace {
 
+template <int G, int BT, int MODE>
+void walk_launch_w(const Bufs& b, const Params& p, hipStream_t s) {
+    k_walk<G, BT, MODE><<<walk_blocks(b.ngroups, G, BT), BT, 0, s>>>(
+        b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev, b.f,
+        b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
+}
+
 template <int G, int BT>
 void walk_launch(const Bufs& b, const Params& p, hipStream_t s) {
-    if (p.walk_window)
-        k_walk<G, BT, true><<<walk_blocks(b.ngroups, G, BT), BT, 0, s>>>(
-            b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev,
-            b.f, b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
-    else
-        k_walk<G, BT, false><<<walk_blocks(b.ngroups, G, BT), BT, 0, s>>>(
-            b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev,
-            b.f, b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
+    switch (p.walk_fetch) {
+        case 1: walk_launch_w<G, BT, 1>(b, p, s); break;
+        case 2: walk_launch_w<G, BT, 2>(b, p, s); break;
+        default: walk_launch_w<G, BT, 0>(b, p, s); break;
+    }
 }
 
 template <int G>

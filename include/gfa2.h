@@ -154,7 +154,9 @@ int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** nam
 typedef struct gfa2_launch_config {
     int32_t walk_block;         /* walk threads per block: 64, 128, 256 or 512 */
     int32_t blocks_per_cu;      /* grid of the grid-stride kernels, per compute unit */
-    int32_t walk_window;        /* 1: the walk loads tree records a group-wide window at a time */
+    int32_t walk_fetch;         /* how the walk fetches tree records: 0 one load per step, 1 a
+                                   group-wide window of consecutive records read by lane shuffles,
+                                   2 records in pairs (the second used when the walk moves on by one) */
     int32_t reserved;
     double ms_per_iteration;    /* measured by gfa2_autotune */
 } gfa2_launch_config;

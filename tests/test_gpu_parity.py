@@ -208,7 +208,8 @@ def test_coincident_bodies_are_finite():
 
 @pytest.mark.parametrize("walk_block,blocks_per_cu,window,group", [
     (64, 1, 0, 32), (128, 2, 0, 32), (512, 8, 0, 32), (256, 3, 0, 32),
-    (64, 8, 1, 32), (256, 4, 1, 32), (512, 1, 1, 32), (128, 8, 1, 64), (256, 2, 0, 64)])
+    (64, 8, 1, 32), (256, 4, 1, 32), (512, 1, 1, 32), (128, 8, 1, 64), (256, 2, 0, 64),
+    (64, 8, 2, 32), (256, 4, 2, 32), (128, 8, 2, 64), (512, 2, 2, 64)])
 def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, window, group):
     print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu} window={window} "
           f"group={group}", flush=True)
@@ -219,7 +220,7 @@ def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, window, 
         if shape is not None:
             eng.set_launch_config(*shape)
             assert eng.launch_config()["walk_block"] == walk_block
-            assert eng.launch_config()["walk_window"] == window
+            assert eng.launch_config()["walk_fetch"] == window
         eng.set_positions(pos)
         eng.step(20)
         eng.sync()
