@@ -34,6 +34,7 @@ ABI_SYMBOLS = (
     "gfa2_stress_setup", "gfa2_stress_pairs", "gfa2_normalized_stress",
     "gfa2_set_graph_batch", "gfa2_get_batch_stats",
     "gfa2_autotune", "gfa2_get_launch_config", "gfa2_set_launch_config", "gfa2_tuned_config",
+    "gfa2_set_edge_weights", "gfa2_set_graph_coo_weighted", "gfa2_exact_repulsion",
 )
 
 
@@ -51,6 +52,9 @@ class Params(ctypes.Structure):
         ("speed_efficiency", ctypes.c_float),
         ("group_size", ctypes.c_int32),
         ("cell_factor", ctypes.c_int32),
+        ("linlog", ctypes.c_int32),
+        ("edge_weight_influence", ctypes.c_float),
+        ("prevent_overlap", ctypes.c_int32),
     ]
 
 
@@ -142,6 +146,9 @@ def lib() -> ctypes.CDLL:
     L.gfa2_get_launch_config.argtypes = [_P, ctypes.POINTER(LaunchConfig)]
     L.gfa2_set_launch_config.argtypes = [_P, ctypes.POINTER(LaunchConfig)]
     L.gfa2_tuned_config.argtypes = [_I32, ctypes.POINTER(LaunchConfig)]
+    L.gfa2_set_edge_weights.argtypes = [_P, _I64, _P]
+    L.gfa2_set_graph_coo_weighted.argtypes = [_P, _I32, _I64, _P, _P, _P, _P]
+    L.gfa2_exact_repulsion.argtypes = [_P, _I32, _P]
     for name in ABI_SYMBOLS:
         fn = getattr(L, name)
         if name not in ("gfa2_default_params", "gfa2_strerror"):
@@ -188,7 +195,8 @@ def coo_to_csr(n: int, src: np.ndarray, dst: np.ndarray):
 class Engine:
     """One layout instance on one GPU (gfa2_handle)."""
 
-    def __init__(self, n: int, src=None, dst=None, rowptr=None, col=None, mass=None, device: int = 0, **params):
+    def __init__(self, n: int, src=None, dst=None, rowptr=None, col=None, mass=None, device: int = 0,
+                 weights=None, **params):
         self.params = default_params(**params)
         h = ctypes.c_void_p()
         _check(lib().gfa2_create(ctypes.byref(self.params), device, ctypes.byref(h)), "gfa2_create")
@@ -203,8 +211,13 @@ class Engine:
         else:
             s = np.ascontiguousarray(src, dtype=np.int32)
             d = np.ascontiguousarray(dst, dtype=np.int32)
-            _check(lib().gfa2_set_graph_coo(self.h, self.n, len(s), _ptr(s), _ptr(d), _ptr(m)),
-                   "gfa2_set_graph_coo")
+            if weights is not None:
+                w = np.ascontiguousarray(weights, dtype=np.float32)
+                _check(lib().gfa2_set_graph_coo_weighted(self.h, self.n, len(s), _ptr(s), _ptr(d), _ptr(w), _ptr(m)),
+                       "gfa2_set_graph_coo_weighted")
+            else:
+                _check(lib().gfa2_set_graph_coo(self.h, self.n, len(s), _ptr(s), _ptr(d), _ptr(m)),
+                       "gfa2_set_graph_coo")
 
     def close(self):
         if getattr(self, "h", None):
@@ -295,6 +308,27 @@ class Engine:
         _check(lib().gfa2_normalized_stress(self.h, _ptr(p), ctypes.byref(ns), ctypes.byref(alpha)),
                "gfa2_normalized_stress")
         return ns.value, alpha.value
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if not hasattr(self.params, k):
+                raise KeyError(k)
+            setattr(self.params, k, v)
+        _check(lib().gfa2_set_params(self.h, ctypes.byref(self.params)), "gfa2_set_params")
+
+    def set_edge_weights(self, w: Optional[np.ndarray]):
+        """Weights per CSR entry (None clears them)."""
+        if w is None:
+            _check(lib().gfa2_set_edge_weights(self.h, 0, None), "gfa2_set_edge_weights")
+            return
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().gfa2_set_edge_weights(self.h, len(w), _ptr(w)), "gfa2_set_edge_weights")
+
+    def exact_repulsion(self, fp64: bool = True) -> np.ndarray:
+        """All-pairs repulsion of the current positions (V x 2, float64)."""
+        out = np.empty((self.n, 2), np.float64)
+        _check(lib().gfa2_exact_repulsion(self.h, int(bool(fp64)), _ptr(out)), "gfa2_exact_repulsion")
+        return out
 
     def autotune(self, iters: int = 10) -> dict:
         """Times every launch shape over `iters` iterations and keeps the fastest; the

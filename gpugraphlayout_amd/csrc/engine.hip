@@ -34,6 +34,7 @@ struct gfa2_engine {
     long long edges = 0;
     bool capture_rep = false;
     float2* rep_buf = nullptr;          // repulsion of the last captured iteration (kept when capture stops)
+    float* ew_buf = nullptr;            // w^delta per CSR entry, once weights were given
     std::vector<void*> allocs;
     // timing
     bool timing = false;
@@ -71,6 +72,15 @@ int size_bucket(int n) {
     int b = 0;
     while ((2ll << b) <= n) ++b;
     return b;
+}
+
+// Parameter checks shared by gfa2_create and gfa2_set_params.
+int validate_params(const gfa2_params& p) {
+    if (p.prevent_overlap) return GFA2_ERR_UNSUPPORTED;
+    if (p.group_size != 32 && p.group_size != 64) return GFA2_ERR_INVALID_ARG;
+    if (!(p.theta > 0.0f) || !(p.epssq >= 0.0f)) return GFA2_ERR_INVALID_ARG;
+    if (!std::isfinite(p.edge_weight_influence)) return GFA2_ERR_INVALID_ARG;
+    return GFA2_OK;
 }
 
 void apply_tune(gfa2_engine* h) {
@@ -114,6 +124,7 @@ void free_all(gfa2_engine* h) {
     h->allocs.clear();
     h->b = Bufs{};
     h->rep_buf = nullptr;
+    h->ew_buf = nullptr;
     h->has_graph = false;
     h->has_pos = false;
     h->stress_K = h->stress_T = 0;
@@ -142,6 +153,21 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
     launch_attraction(b, h->p, s);
     if (ev) hipEventRecord(ev[1], s);
+    if (h->p.exact) {
+        // exact repulsion: no keys, sort or tree (their groups record empty intervals)
+        if (ev) {
+            hipEventRecord(ev[2], s);
+            hipEventRecord(ev[3], s);
+            hipEventRecord(ev[4], s);
+        }
+        launch_exact(b, h->p, s);
+        if (ev) hipEventRecord(ev[5], s);
+        launch_speed(b, s);
+        if (ev) hipEventRecord(ev[6], s);
+        launch_displacement(b, s);
+        if (ev) hipEventRecord(ev[7], s);
+        return;
+    }
     if (b.nseg > 1) {
         launch_seg_bbox(b, s);
         launch_seg_keys(b, s);
@@ -230,6 +256,9 @@ void gfa2_default_params(gfa2_params* p) {
     p->speed_efficiency = 1.0f;
     p->group_size = 32;
     p->cell_factor = 0;
+    p->linlog = 0;
+    p->edge_weight_influence = 0.0f;   // src/RPForceAtlas2.cpp:49
+    p->prevent_overlap = 0;
 }
 
 int gfa2_device_info(int device, int32_t* cus, int32_t* wave, int64_t* mem_bytes) {
@@ -250,9 +279,9 @@ int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out) {
     gfa2_params prm;
     if (params) prm = *params;
     else gfa2_default_params(&prm);
-    if (!prm.use_barneshut) return GFA2_ERR_UNSUPPORTED;
-    if (prm.group_size != 32 && prm.group_size != 64) return GFA2_ERR_INVALID_ARG;
-    if (!(prm.theta > 0.0f) || !(prm.epssq >= 0.0f) || prm.cell_factor < 0) return GFA2_ERR_INVALID_ARG;
+    const int valid = validate_params(prm);
+    if (valid != GFA2_OK) return valid;
+    if (prm.cell_factor < 0) return GFA2_ERR_INVALID_ARG;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GFA2_ERR_NO_DEVICE;
     if (device < 0 || device >= count) return GFA2_ERR_INVALID_ARG;
@@ -272,6 +301,8 @@ int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out) {
     h->p.itolsq = 1.0f / (prm.theta * prm.theta);
     h->p.epssq = prm.epssq;
     h->p.group = prm.group_size;
+    h->p.exact = prm.use_barneshut ? 0 : 1;
+    h->p.linlog = prm.linlog ? 1 : 0;
     h->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -284,9 +315,9 @@ int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out) {
 
 int gfa2_set_params(gfa2_handle h, const gfa2_params* prm) {
     if (!h || !prm) return GFA2_ERR_INVALID_ARG;
-    if (!prm->use_barneshut) return GFA2_ERR_UNSUPPORTED;
-    if (prm->group_size != 32 && prm->group_size != 64) return GFA2_ERR_INVALID_ARG;
-    if (!(prm->theta > 0.0f) || !(prm->epssq >= 0.0f)) return GFA2_ERR_INVALID_ARG;
+    const int valid = validate_params(*prm);
+    if (valid != GFA2_OK) return valid;
+    const bool new_delta = prm->edge_weight_influence != h->user.edge_weight_influence;
     const int keep_factor = h->user.cell_factor;
     h->user = *prm;
     h->user.cell_factor = keep_factor;   // capacity is fixed by set_graph
@@ -295,11 +326,19 @@ int gfa2_set_params(gfa2_handle h, const gfa2_params* prm) {
     h->p.k_r = prm->scale;
     h->p.itolsq = 1.0f / (prm->theta * prm->theta);
     h->p.epssq = prm->epssq;
+    h->p.exact = prm->use_barneshut ? 0 : 1;
+    h->p.linlog = prm->linlog ? 1 : 0;
     const bool regroup = h->p.group != prm->group_size;
     h->p.group = prm->group_size;
     if (regroup && h->has_graph) {
         const int rc = build_groups(h);
         if (rc != GFA2_OK) return rc;
+    }
+    if (new_delta && h->b.ew) {
+        GFA2_TRY(hipSetDevice(h->device));
+        launch_edge_weights(h->b, prm->edge_weight_influence, h->stream);
+        GFA2_TRY(hipGetLastError());
+        GFA2_TRY(hipStreamSynchronize(h->stream));
     }
     h->graph_valid = false;
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
@@ -733,6 +772,76 @@ int gfa2_set_graph_capture(gfa2_handle h, int32_t on) {
     if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
     return GFA2_OK;
+}
+
+int gfa2_set_edge_weights(gfa2_handle h, int64_t count, const float* w) {
+    if (!h) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
+    Bufs& b = h->b;
+    GFA2_TRY(hipSetDevice(h->device));
+    if (!w) {
+        b.ew = nullptr;                  // buffers stay allocated until the next graph
+    } else {
+        if (count != b.nnz) return GFA2_ERR_INVALID_ARG;
+        for (int64_t e = 0; e < count; ++e)
+            if (!(w[e] > 0.0f) || !std::isfinite(w[e])) return GFA2_ERR_INVALID_ARG;
+        if (!b.ew_raw) {
+            int st = dalloc(h, &b.ew_raw, (size_t)b.nnz);
+            if (st == GFA2_OK) st = dalloc(h, &h->ew_buf, (size_t)b.nnz);
+            if (st != GFA2_OK) return st;
+        }
+        GFA2_TRY(hipMemcpyAsync(b.ew_raw, w, sizeof(float) * (size_t)count, hipMemcpyHostToDevice, h->stream));
+        b.ew = h->ew_buf;
+        launch_edge_weights(b, h->user.edge_weight_influence, h->stream);
+        GFA2_TRY(hipGetLastError());
+        GFA2_TRY(hipStreamSynchronize(h->stream));
+    }
+    h->graph_valid = false;
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    return GFA2_OK;
+}
+
+int gfa2_set_graph_coo_weighted(gfa2_handle h, int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                                const float* weight, const float* mass) {
+    if (!h || n < 1 || m < 0 || (m > 0 && (!src || !dst || !weight)))
+        return n < 1 ? GFA2_ERR_INVALID_GRAPH : GFA2_ERR_INVALID_ARG;
+    std::vector<int64_t> rowptr((size_t)n + 1, 0);
+    for (int64_t e = 0; e < m; ++e) {
+        if (src[e] < 0 || src[e] >= n || dst[e] < 0 || dst[e] >= n || src[e] == dst[e]) return GFA2_ERR_INVALID_GRAPH;
+        rowptr[src[e] + 1]++;
+        rowptr[dst[e] + 1]++;
+    }
+    for (int v = 0; v < n; ++v) rowptr[v + 1] += rowptr[v];
+    std::vector<int64_t> cur(rowptr.begin(), rowptr.end() - 1);
+    std::vector<int32_t> col((size_t)(2 * m));
+    std::vector<float> w((size_t)(2 * m));
+    for (int64_t e = 0; e < m; ++e) {
+        w[cur[src[e]]] = weight[e];
+        col[cur[src[e]]++] = dst[e];
+        w[cur[dst[e]]] = weight[e];
+        col[cur[dst[e]]++] = src[e];
+    }
+    int st = gfa2_set_graph(h, n, rowptr.data(), col.data(), mass);
+    if (st != GFA2_OK) return st;
+    return gfa2_set_edge_weights(h, 2 * m, w.data());
+}
+
+int gfa2_exact_repulsion(gfa2_handle h, int32_t fp64, double* xy) {
+    if (!h || !xy) return GFA2_ERR_INVALID_ARG;
+    if (!h->has_graph || !h->has_pos) return GFA2_ERR_NO_GRAPH;
+    GFA2_TRY(hipSetDevice(h->device));
+    int rc = drain_timing(h);
+    if (rc != GFA2_OK) return rc;
+    double2* out = nullptr;
+    GFA2_TRY(hipMalloc(&out, sizeof(double2) * (size_t)h->b.n));
+    launch_exact_only(h->b, h->p, fp64 != 0, out, h->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(xy, out, sizeof(double2) * (size_t)h->b.n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(out);
+    return e == hipSuccess ? GFA2_OK : hip_status(e);
 }
 
 int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c) {

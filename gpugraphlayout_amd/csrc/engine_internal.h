@@ -80,6 +80,8 @@ struct Params {
     float k_g, k_r;
     float itolsq, epssq;
     int group;              // walk vote group: 32 or 64 lanes
+    int exact = 0;          // 1: exact O(V^2) repulsion instead of the tree and walk
+    int linlog = 0;         // attraction log(1+d)/d per unit vector
     int walk_bt = 256;      // walk threads per block (64..512); results do not depend on it
     int walk_fetch = 0;     // walk record fetch mode (0 direct, 1 window, 2 paired); no result depends on it
 };
@@ -113,7 +115,9 @@ struct Bufs {
     int* S;
     float4* rec;
     int* brec;
-    float4* sbody;          // {x, y, mass, 0} of the body at sorted position k
+    float4* sbody;          // {x, y, mass, 0} of the body at sorted position k (node k in exact mode)
+    float* ew_raw = nullptr;    // edge weights per CSR entry (null: unweighted)
+    float* ew = nullptr;        // w^delta per CSR entry
     int* rcount;
     int* cell_rec;
     unsigned char* cell_lvl;
@@ -180,6 +184,13 @@ void launch_tree(Bufs& b, hipStream_t s);              // after the sort: cells 
 void launch_walk(const Bufs& b, const Params& p, hipStream_t s);
 void launch_speed(const Bufs& b, hipStream_t s);
 void launch_displacement(const Bufs& b, hipStream_t s);
+// Exact mode: pack {pos, mass} in node order, then all-pairs repulsion per graph with the
+// walk's epilogue (total force, swing/traction partials per group).
+void launch_exact(const Bufs& b, const Params& p, hipStream_t s);
+// All-pairs repulsion of the current positions into out (double2 per node), FP32 or FP64 sums.
+void launch_exact_only(const Bufs& b, const Params& p, bool fp64, double2* out, hipStream_t s);
+// w^delta of every CSR entry.
+void launch_edge_weights(const Bufs& b, float delta, hipStream_t s);
 
 // Layout quality (stress.hip): hop distances of K*T sampled pairs by multi-source BFS,
 // and the normalized-stress sums {sum r, sum r^2, pairs} for positions pos.

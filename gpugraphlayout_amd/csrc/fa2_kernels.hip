@@ -25,11 +25,37 @@ __device__ __forceinline__ float2 gravity(float2 p, float m, int strong, float k
     return make_float2(-p.x * fg, -p.y * fg);
 }
 
+// One edge's attraction term on node p from neighbour q. Linear: q - p. With linlog (the
+// reference's CPU rule, src/RPCPUForceAtlas2.cpp:63-67): (q - p) * log(1 + d) / d, and 0 at
+// d = 0 (the reference multiplies the zero vector by FLT_MAX there). With edge weights:
+// times w^delta.
+template <bool GEN>
+__device__ __forceinline__ void attract_term(float2 p, float2 q, int linlog, const float* __restrict__ ew, int e,
+                                             float& ax, float& ay) {
+    const float dx = q.x - p.x;
+    const float dy = q.y - p.y;
+    if constexpr (!GEN) {
+        ax += dx;
+        ay += dy;
+    } else {
+        float fa = 1.0f;
+        if (linlog) {
+            const float d = sqrtf(dx * dx + dy * dy);
+            fa = d == 0.0f ? 0.0f : logf(1.0f + d) / d;
+        }
+        if (ew) fa *= ew[e];
+        ax += dx * fa;
+        ay += dy * fa;
+    }
+}
+
+template <bool GEN>
 __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos, const float* __restrict__ mass,
                                                  const int* __restrict__ rowptr, const int* __restrict__ col,
                                                  const int* __restrict__ lists, int nlight, int nmedium,
                                                  int nheavy, int light_blocks, int medium_blocks,
-                                                 int strong, float k_g, float2* __restrict__ fga) {
+                                                 int strong, float k_g, float2* __restrict__ fga, int linlog,
+                                                 const float* __restrict__ ew) {
     __shared__ float2 red[256];
     const int tid = threadIdx.x;
     int blk = blockIdx.x;
@@ -41,9 +67,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         float ax = 0.0f, ay = 0.0f;
         const int e1 = rowptr[v + 1];
         for (int e = rowptr[v]; e < e1; ++e) {
-            const float2 q = pos[col[e]];
-            ax += q.x - p.x;
-            ay += q.y - p.y;
+            attract_term<GEN>(p, pos[col[e]], linlog, ew, e, ax, ay);
         }
         const float2 g = gravity(p, mass[v], strong, k_g);
         fga[v] = make_float2(g.x + ax, g.y + ay);
@@ -59,9 +83,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         float ax = 0.0f, ay = 0.0f;
         const int e1 = rowptr[v + 1];
         for (int e = rowptr[v] + lane; e < e1; e += 64) {
-            const float2 q = pos[col[e]];
-            ax += q.x - p.x;
-            ay += q.y - p.y;
+            attract_term<GEN>(p, pos[col[e]], linlog, ew, e, ax, ay);
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
@@ -81,9 +103,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
     float ax = 0.0f, ay = 0.0f;
     const int e1 = rowptr[v + 1];
     for (int e = rowptr[v] + tid; e < e1; e += 256) {
-        const float2 q = pos[col[e]];
-        ax += q.x - p.x;
-        ay += q.y - p.y;
+        attract_term<GEN>(p, pos[col[e]], linlog, ew, e, ax, ay);
     }
     red[tid] = make_float2(ax, ay);
     __syncthreads();
@@ -119,6 +139,56 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
 // The epilogue adds gravity + attraction, writes the total force, and forms the group's
 // swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
 // ---------------------------------------------------------------------------------------
+// The end of a body's force evaluation, shared by the walk and the exact kernel: total force
+// F = F_ga + F_r, optional repulsion capture, and the group's swing/traction/interaction sums
+// (SpeedKernel's per-body terms, src/RPFA2Kernels.cu:111-161), reduced within the G lanes.
+template <int G>
+__device__ __forceinline__ void group_finish(bool active, bool group_ok, int gl, int gidx, unsigned int id, float mi,
+                                             float ax, float ay, unsigned int inter,
+                                             const float2* __restrict__ fga, const float2* __restrict__ fprev,
+                                             float2* __restrict__ f, float2* __restrict__ rep,
+                                             WalkPart* __restrict__ part) {
+    float sw = 0.0f, tr = 0.0f;
+    if (active) {
+        const float2 a = fga[id];
+        const float2 F = make_float2(a.x + ax, a.y + ay);
+        f[id] = F;
+        if (rep) rep[id] = make_float2(ax, ay);
+        const float2 P = fprev[id];
+        float dx = F.x - P.x, dy = F.y - P.y;
+        sw = mi * sqrtf(dx * dx + dy * dy);
+        dx = F.x + P.x;
+        dy = F.y + P.y;
+        tr = mi * (sqrtf(dx * dx + dy * dy) * 0.5f);
+    }
+    // Reduce within the group (xor offsets below G stay inside it).
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        sw += __shfl_xor(sw, o, 64);
+        tr += __shfl_xor(tr, o, 64);
+        inter += __shfl_xor(inter, o, 64);
+    }
+    if (group_ok && gl == 0) {
+        WalkPart wp;
+        wp.swing = sw;
+        wp.traction = tr;
+        wp.interactions = inter;
+        wp.pad = 0;
+        part[gidx] = wp;
+    }
+}
+
+// XCD-aware block order: consecutive logical blocks stay on one XCD's L2 (blocks are
+// dealt to the 8 XCDs round-robin by hardware block id).
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x;
+    const int xcd = blockIdx.x & 7;
+    const int per = (nb + 7) >> 3;
+    const int full = nb & 7;
+    if (full == 0) return xcd * per + (blockIdx.x >> 3);
+    return (xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1)) + (blockIdx.x >> 3);
+}
+
 // One walk step on record c for the group (G lanes): the vote, the accumulation and the
 // advance of the record index (1, or the subtree size of an accepted cell). A step that is
 // not live contributes nothing: fma(d, 0, a) == a, since a is never -0 here.
@@ -167,14 +237,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     if (st->err != 0) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    // XCD-aware block order: consecutive groups stay on one XCD's L2.
-    const int nb = gridDim.x;
-    const int xcd = blockIdx.x & 7;
-    const int per = (nb + 7) >> 3;
-    const int full = nb & 7;
-    int lb;
-    if (full == 0) lb = xcd * per + (blockIdx.x >> 3);
-    else lb = (xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1)) + (blockIdx.x >> 3);
+    const int lb = xcd_block();
     const int gidx = (lb * BT + tid) / G;
     const int gl = (lb * BT + tid) % G;           // lane within the group
     const bool group_ok = gidx < ngroups;
@@ -234,34 +297,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
             while (r < end) r += walk_step<G>(rec[r], true, active, px, py, kmi, base, epssq, ax, ay, inter);
         }
     }
-    float sw = 0.0f, tr = 0.0f;
-    if (active) {
-        const float2 a = fga[id];
-        const float2 F = make_float2(a.x + ax, a.y + ay);
-        f[id] = F;
-        if (rep) rep[id] = make_float2(ax, ay);
-        const float2 P = fprev[id];
-        float dx = F.x - P.x, dy = F.y - P.y;
-        sw = mi * sqrtf(dx * dx + dy * dy);
-        dx = F.x + P.x;
-        dy = F.y + P.y;
-        tr = mi * (sqrtf(dx * dx + dy * dy) * 0.5f);
-    }
-    // Reduce within the group (xor offsets below G stay inside it).
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-        sw += __shfl_xor(sw, o, 64);
-        tr += __shfl_xor(tr, o, 64);
-        inter += __shfl_xor(inter, o, 64);
-    }
-    if (group_ok && gl == 0) {
-        WalkPart wp;
-        wp.swing = sw;
-        wp.traction = tr;
-        wp.interactions = inter;
-        wp.pad = 0;
-        part[gidx] = wp;
-    }
+    group_finish<G>(active, group_ok, gl, gidx, id, mi, ax, ay, inter, fga, fprev, f, rep, part);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -402,6 +438,101 @@ __global__ __launch_bounds__(256) void k_displace(float2* __restrict__ pos, cons
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Exact repulsion (no counterpart in the reference's GPU path). Each lane is one node; the
+// group's G lanes run over all nodes j of their graph in ascending order, reading the packed
+// {x, y, mass} of node j (one address per group: a broadcast load), with the walk's term
+// k_r m_i m_j (p_i - p_j) / (|p_i - p_j|^2 + eps^2) (node i itself adds 0). Every body
+// interacts with every body of its graph, so the interaction count is the sum of n_g^2.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pack(const float2* __restrict__ pos, const float* __restrict__ mass, int n,
+                                              float4* __restrict__ body) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float2 p = pos[i];
+        body[i] = make_float4(p.x, p.y, mass[i], 0.0f);
+    }
+}
+
+template <int G, int BT>
+__global__ __launch_bounds__(BT) void k_exact(const float4* __restrict__ body, const int2* __restrict__ groups,
+                                              int ngroups, const int* __restrict__ node_off,
+                                              const float2* __restrict__ fga, const float2* __restrict__ fprev,
+                                              float2* __restrict__ f, float2* __restrict__ rep,
+                                              WalkPart* __restrict__ part, float k_r, float epssq,
+                                              const DevState* __restrict__ st) {
+    if (st->err != 0) return;
+    const int lb = xcd_block();
+    const int gidx = (lb * BT + (int)threadIdx.x) / G;
+    const int gl = (lb * BT + (int)threadIdx.x) % G;
+    const bool group_ok = gidx < ngroups;
+    int2 grp = make_int2(0, 0);
+    if (group_ok) grp = groups[gidx];
+    const int g = grp.y;
+    const int k = grp.x + gl;
+    const bool active = group_ok && k < node_off[g + 1];
+    float px = 0.0f, py = 0.0f, mi = 0.0f, kmi = 0.0f;
+    if (active) {
+        const float4 me = body[k];
+        px = me.x;
+        py = me.y;
+        mi = me.z;
+        kmi = k_r * mi;
+    }
+    float ax = 0.0f, ay = 0.0f;
+    unsigned int inter = 0;
+    if (group_ok) {
+        const int j0 = node_off[g], j1 = node_off[g + 1];
+        for (int j = j0; j < j1; ++j) {
+            const float4 c = body[j];
+            const float dx = px - c.x;
+            const float dy = py - c.y;
+            const float d2 = dx * dx + dy * dy + epssq;
+            const float w = kmi * c.z * __builtin_amdgcn_rcpf(d2);
+            ax = fmaf(dx, w, ax);
+            ay = fmaf(dy, w, ay);
+        }
+        inter = active ? (unsigned int)(j1 - j0) : 0u;
+    }
+    group_finish<G>(active, group_ok, gl, gidx, (unsigned int)k, mi, ax, ay, inter, fga, fprev, f, rep, part);
+}
+
+// All-pairs repulsion into out, one thread per node, FP32 or FP64 arithmetic; the graph of
+// node i is found from node_off by binary search.
+template <typename T>
+__global__ __launch_bounds__(256) void k_exact_only(const float4* __restrict__ body, int n, int nseg,
+                                                    const int* __restrict__ node_off, float k_r, float epssq,
+                                                    double2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nseg;           // node_off[lo] <= i < node_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (node_off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    const float4 me = body[i];
+    const T px = me.x, py = me.y, kmi = (T)k_r * (T)me.z, eps = epssq;
+    T ax = 0, ay = 0;
+    for (int j = node_off[lo]; j < node_off[lo + 1]; ++j) {
+        const float4 c = body[j];
+        const T dx = px - (T)c.x;
+        const T dy = py - (T)c.y;
+        const T d2 = dx * dx + dy * dy + eps;
+        const T w = kmi * (T)c.z / d2;
+        ax += dx * w;
+        ay += dy * w;
+    }
+    out[i] = make_double2((double)ax, (double)ay);
+}
+
+// w^delta per CSR entry (delta = 0 gives 1).
+__global__ __launch_bounds__(256) void k_edge_pow(const float* __restrict__ w, long long nnz, float delta,
+                                                  float* __restrict__ out) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nnz;
+         e += (long long)gridDim.x * blockDim.x)
+        out[e] = delta == 0.0f ? 1.0f : (delta == 1.0f ? w[e] : powf(w[e], delta));
+}
+
 }  // namespace
 
 int walk_blocks(int ngroups, int group, int bt) {
@@ -445,8 +576,12 @@ void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
     const int mb = (b.nmedium + 3) / 4;
     const int total = lb + mb + b.nheavy;
     if (total == 0) return;
-    k_attract<<<total, 256, 0, s>>>(b.pos, b.mass, b.rowptr, b.col, b.node_lists, b.nlight, b.nmedium, b.nheavy,
-                                    lb, mb, p.strong_gravity, p.k_g, b.fga);
+    if (p.linlog || b.ew)
+        k_attract<true><<<total, 256, 0, s>>>(b.pos, b.mass, b.rowptr, b.col, b.node_lists, b.nlight, b.nmedium,
+                                              b.nheavy, lb, mb, p.strong_gravity, p.k_g, b.fga, p.linlog, b.ew);
+    else
+        k_attract<false><<<total, 256, 0, s>>>(b.pos, b.mass, b.rowptr, b.col, b.node_lists, b.nlight, b.nmedium,
+                                               b.nheavy, lb, mb, p.strong_gravity, p.k_g, b.fga, 0, nullptr);
 }
 
 void launch_walk(const Bufs& b, const Params& p, hipStream_t s) {
@@ -461,6 +596,39 @@ void launch_speed(const Bufs& b, hipStream_t s) {
 void launch_displacement(const Bufs& b, hipStream_t s) {
     k_displace<<<displacement_blocks(b), 256, 0, s>>>(b.pos, b.f, b.fprev, b.n, b.nseg > 1 ? b.seg_of : nullptr,
                                                        b.seg_st, b.nseg > 1 ? nullptr : b.bbox_part, b.st);
+}
+
+void launch_exact(const Bufs& b, const Params& p, hipStream_t s) {
+    int pb = (b.n + 255) / 256;
+    if (pb > 4096) pb = 4096;
+    k_pack<<<pb, 256, 0, s>>>(b.pos, b.mass, b.n, b.sbody);
+    constexpr int BT = 256;
+    if (p.group == 64)
+        k_exact<64, BT><<<walk_blocks(b.ngroups, 64, BT), BT, 0, s>>>(b.sbody, b.groups, b.ngroups, b.node_off, b.fga,
+                                                                      b.fprev, b.f, b.rep, b.grp_part, p.k_r,
+                                                                      p.epssq, b.st);
+    else
+        k_exact<32, BT><<<walk_blocks(b.ngroups, 32, BT), BT, 0, s>>>(b.sbody, b.groups, b.ngroups, b.node_off, b.fga,
+                                                                      b.fprev, b.f, b.rep, b.grp_part, p.k_r,
+                                                                      p.epssq, b.st);
+}
+
+void launch_exact_only(const Bufs& b, const Params& p, bool fp64, double2* out, hipStream_t s) {
+    int pb = (b.n + 255) / 256;
+    if (pb > 4096) pb = 4096;
+    k_pack<<<pb, 256, 0, s>>>(b.pos, b.mass, b.n, b.sbody);
+    const int blocks = (b.n + 255) / 256;
+    if (fp64)
+        k_exact_only<double><<<blocks, 256, 0, s>>>(b.sbody, b.n, b.nseg, b.node_off, p.k_r, p.epssq, out);
+    else
+        k_exact_only<float><<<blocks, 256, 0, s>>>(b.sbody, b.n, b.nseg, b.node_off, p.k_r, p.epssq, out);
+}
+
+void launch_edge_weights(const Bufs& b, float delta, hipStream_t s) {
+    if (!b.ew_raw || !b.ew || b.nnz == 0) return;
+    long long blocks = (b.nnz + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    k_edge_pow<<<(int)blocks, 256, 0, s>>>(b.ew_raw, b.nnz, delta, b.ew);
 }
 
 }  // namespace gfa2

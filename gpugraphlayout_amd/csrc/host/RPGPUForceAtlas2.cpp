@@ -28,6 +28,8 @@ namespace RPGraph
         params.theta = theta;
         params.epssq = epssq;
         params.group_size = group_size;
+        params.linlog = use_linlog ? 1 : 0;
+        params.prevent_overlap = prevent_overlap ? 1 : 0;
         check(gfa2_create(&params, device, &h), "gfa2_create");
         std::vector<int32_t> src, dst;
         layout.graph.coo(src, dst);
@@ -49,8 +51,21 @@ namespace RPGraph
         if (h) gfa2_destroy(h);
     }
 
+    void GPUForceAtlas2::push_flags()
+    {
+        const int32_t bh = use_barneshut ? 1 : 0, ll = use_linlog ? 1 : 0, po = prevent_overlap ? 1 : 0;
+        if (bh == params.use_barneshut && ll == params.linlog && po == params.prevent_overlap) return;
+        gfa2_params next = params;
+        next.use_barneshut = bh;
+        next.linlog = ll;
+        next.prevent_overlap = po;
+        check(gfa2_set_params(h, &next), "gfa2_set_params");
+        params = next;
+    }
+
     void GPUForceAtlas2::doStep()
     {
+        push_flags();
         check(gfa2_step(h, 1), "gfa2_step");
         check(gfa2_sync(h), "gfa2_sync");
         iteration++;
@@ -59,6 +74,7 @@ namespace RPGraph
     void GPUForceAtlas2::doSteps(int n)
     {
         if (n <= 0) return;
+        push_flags();
         check(gfa2_step(h, n), "gfa2_step");
         check(gfa2_sync(h), "gfa2_sync");
         iteration += n;

@@ -42,6 +42,9 @@ namespace RPGraph
         gfa2_handle h = nullptr;
         gfa2_params params;
         void check(int rc, const char* what);
+        // Pushes the public ForceAtlas2 flags (use_barneshut: exact repulsion when false,
+        // use_linlog, prevent_overlap: unsupported) to the engine when a caller changed them.
+        void push_flags();
     };
 
     // The reference's class name, for callers written against it.

@@ -66,11 +66,6 @@ int main(int argc, const char** argv)
         else if (arg == "bin") out_format = "bin";
     }
 
-    if (gpu_requested && !approximate)
-    {
-        fprintf(stderr, "error: The GPU implementation requires Barnes-Hut approximation.\n");
-        return EXIT_FAILURE;
-    }
     if (!gpu_requested)
     {
         fprintf(stderr, "error: only the GPU engine is built here (use `gpu`).\n");

@@ -33,7 +33,9 @@ typedef struct gfa2_engine* gfa2_handle;
 /* FA2 parameters. Defaults (gfa2_default_params) are the reference's
  * (src/RPForceAtlas2.cpp:28-55). */
 typedef struct gfa2_params {
-    int32_t use_barneshut;      /* must be 1 (the GPU path is Barnes-Hut only) */
+    int32_t use_barneshut;      /* 1: Barnes-Hut repulsion; 0: exact O(V^2) repulsion (all pairs, the
+                                   same term and epsilon; the reference's GPU path has no exact mode,
+                                   src/graph_viewer.cpp:93-97) */
     int32_t strong_gravity;     /* sg / wg, src/graph_viewer.cpp:62 */
     float gravity;              /* k_g, ForceAtlas2::setGravity, src/RPForceAtlas2.cpp:69-72 */
     float scale;                /* k_r, ForceAtlas2::setScale, src/RPForceAtlas2.cpp:64-67 */
@@ -45,6 +47,12 @@ typedef struct gfa2_params {
     float speed_efficiency;     /* initial efficiency, 1.0 */
     int32_t group_size;         /* walk vote group: 32 = the reference's warp vote; 64 = wave64 */
     int32_t cell_factor;        /* cell capacity = cell_factor * V (+ slack); 0 = default 3 */
+    int32_t linlog;             /* ForceAtlas2::use_linlog: attraction log(1+d)/d per unit vector,
+                                   as src/RPCPUForceAtlas2.cpp:63-67 (0 = linear) */
+    float edge_weight_influence;/* delta (src/RPForceAtlas2.hpp:49): attraction scaled by w^delta
+                                   for edge weight w (gfa2_set_edge_weights); 0 = unweighted */
+    int32_t prevent_overlap;    /* ForceAtlas2::prevent_overlap: must be 0 (GFA2_ERR_UNSUPPORTED,
+                                   as the reference exits, src/RPCPUForceAtlas2.cpp:201-205) */
 } gfa2_params;
 
 /* Per-iteration statistics (no counterpart in the reference, which reads none back). */
@@ -169,6 +177,21 @@ int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c);
 int gfa2_set_launch_config(gfa2_handle h, const gfa2_launch_config* c);
 /* The tuned shapes recorded for the size bucket of `nodes`; GFA2_ERR_NO_GRAPH if none. */
 int gfa2_tuned_config(int32_t nodes, gfa2_launch_config* c);
+
+/* Edge weights w (> 0, finite) for the attraction, one per CSR entry in the order of the col
+ * array given to gfa2_set_graph or gfa2_set_graph_batch (count == rowptr[n]; the two entries
+ * of an undirected edge should carry the same weight). NULL clears them. Used with
+ * params.edge_weight_influence (w^delta); no counterpart in the reference, which declares
+ * delta but never reads it. */
+int gfa2_set_edge_weights(gfa2_handle h, int64_t count, const float* w);
+/* gfa2_set_graph_coo with one weight per COO edge (the CSR order is built here). */
+int gfa2_set_graph_coo_weighted(gfa2_handle h, int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                                const float* weight, const float* mass);
+
+/* Exact repulsion of the current positions, all pairs of each graph with the same term as the
+ * walk (k_r m_i m_j d / (|d|^2 + eps^2)), summed in FP32 (fp64 = 0) or FP64 (fp64 = 1), j in
+ * ascending order; xy gets V (x, y) pairs. Does not advance the layout. */
+int gfa2_exact_repulsion(gfa2_handle h, int32_t fp64, double* xy);
 
 /* Graph capture of the whole iteration (hipGraph), replayed per iteration. */
 int gfa2_set_graph_capture(gfa2_handle h, int32_t on);

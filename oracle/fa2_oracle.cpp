@@ -310,6 +310,30 @@ void orc_attraction_coo(long long m, const int* src, const int* dst, const float
     }
 }
 
+// Attraction with the options the reference declares: linlog as its CPU backend computes it
+// (src/RPCPUForceAtlas2.cpp:55-82: f_a/d = log(1+d)/d, and the zero vector times FLT_MAX,
+// i.e. nothing, at d = 0) and edge weights w^delta (delta: src/RPForceAtlas2.hpp:49, read
+// nowhere in the reference; Gephi's ForceAtlas2 multiplies the attraction by w^delta). w may
+// be null (weight 1). Applied in COO order like orc_attraction_coo.
+void orc_attraction_general(long long m, const int* src, const int* dst, const float* w, float delta, int linlog,
+                            const float* pos, float* f) {
+    for (long long e = 0; e < m; ++e) {
+        int s = src[e], t = dst[e];
+        float dx = pos[2 * t] - pos[2 * s];
+        float dy = pos[2 * t + 1] - pos[2 * s + 1];
+        float fa = 1.0f;
+        if (linlog) {
+            float d = std::sqrt(dx * dx + dy * dy);
+            fa = d == 0.0f ? 0.0f : std::log(1.0f + d) / d;
+        }
+        if (w) fa *= std::pow(w[e], delta);
+        f[2 * s] += dx * fa;
+        f[2 * s + 1] += dy * fa;
+        f[2 * t] += -(dx * fa);
+        f[2 * t + 1] += -(dy * fa);
+    }
+}
+
 // Bounding box: out[0]=root x, out[1]=root y, out[2]=radius.
 void orc_bbox(int n, const float* pos, float* out) { bbox(n, pos, &out[0], &out[1], &out[2]); }
 

@@ -90,6 +90,15 @@ def attraction_coo(pos, src, dst, f=None):
     return f
 
 
+def attraction_general(pos, src, dst, w=None, delta=0.0, linlog=False, f=None):
+    n = len(pos)
+    f = np.zeros((n, 2), np.float32) if f is None else f32(f).copy()
+    wp = None if w is None else f32(w)
+    lib().orc_attraction_general(ctypes.c_longlong(len(src)), _p(i32(src)), _p(i32(dst)), _p(wp),
+                                 ctypes.c_float(delta), int(bool(linlog)), _p(f32(pos)), _p(f))
+    return f
+
+
 def bbox(pos):
     out = np.zeros(3, np.float32)
     lib().orc_bbox(len(pos), _p(f32(pos)), _p(out))

@@ -41,6 +41,9 @@ def test_library_host_only_calls():
     bad = g2.default_params(group_size=48)
     assert lib.gfa2_create(ctypes.byref(bad), 0, ctypes.byref(h)) != 0
     assert lib.gfa2_step(None, 1) == -3
+    overlap = g2.default_params(prevent_overlap=1)
+    assert lib.gfa2_create(ctypes.byref(overlap), 0, ctypes.byref(h)) == -8   # unsupported, as the reference
+    assert lib.gfa2_exact_repulsion(None, 1, None) == -3
     c = g2.LaunchConfig()
     assert lib.gfa2_tuned_config(0, ctypes.byref(c)) == -3
     assert lib.gfa2_tuned_config(1000, ctypes.byref(c)) == -7   # nothing tuned in this process

@@ -21,8 +21,6 @@ def run(args):
 def test_usage_and_rejections(tmp_path):
     r = run(["gpu", "10"])
     assert r.returncode != 0 and "Usage" in r.stderr
-    r = run(["gpu", "10", "1", "wg", "1", "1", "exact", EDGES, str(tmp_path), "csv"])
-    assert r.returncode != 0 and "Barnes-Hut" in r.stderr
     r = run(["gpu", "10", "1", "wg", "1", "1", "approximate", str(tmp_path / "none.txt"), str(tmp_path), "csv"])
     assert r.returncode != 0 and "No edgelist" in r.stderr
     r = run(["gpu", "10", "1", "wg", "1", "1", "approximate", EDGES, str(tmp_path / "nodir"), "csv"])
@@ -59,3 +57,25 @@ def test_graph_viewer_matches_engine(tmp_path):
     r = run(["gpu", "10", "1", "sg", "2", "1", "approximate", EDGES, str(tmp_path), "png", "300", "200"])
     assert r.returncode == 0, r.stderr
     assert open(tmp_path / "random_a.txt_10.png", "rb").read()[:4] == b"\x89PNG"
+
+
+@pytest.mark.gpu
+def test_graph_viewer_exact_mode_matches_engine(tmp_path):
+    """`exact` on the GPU (the reference's GPU backend refuses it) runs the all-pairs
+    repulsion; the CLI's layout equals the engine's in exact mode bit for bit."""
+    print("case graph_viewer exact", flush=True)
+    r = run(["gpu", "20", "1", "wg", "5", "1", "exact", EDGES, str(tmp_path), "bin"])
+    assert r.returncode == 0, r.stderr
+    raw = open(tmp_path / "random_a.txt_20.bin", "rb").read()
+    recs = np.array([struct.unpack("<Iff", raw[12 * k:12 * k + 12]) for k in range(len(raw) // 12)])
+    xy_cli = recs[:, 1:].astype(np.float32)
+    from gpugraphlayout_amd import Engine
+    g = graphs.load_edgelist(EDGES)
+    pos = graphs.glibc_random_positions(g.n, 10000.0, 10000.0, seed=1234)
+    eng = Engine(g.n, g.src, g.dst, mass=g.mass(), scale=5.0, gravity=1.0, use_barneshut=0)
+    eng.set_positions(pos)
+    eng.step(20)
+    eng.sync()
+    xy = eng.positions()
+    eng.close()
+    np.testing.assert_array_equal(xy_cli.view(np.uint32), xy.view(np.uint32))

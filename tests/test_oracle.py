@@ -398,3 +398,34 @@ def test_normalized_stress(oracle):
     ns0, _ = oracle.normalized_stress(np.zeros(4, np.int32), np.arange(1, 5, dtype=np.int32),
                                       np.arange(1, 5, dtype=np.int32), line)
     assert ns0 == pytest.approx(0.0, abs=1e-12)
+
+
+def test_attraction_general_closed_forms(oracle):
+    pos = np.array([[0.0, 0.0], [3.0, 0.0]], np.float32)
+    s, t = np.array([0]), np.array([1])
+    # linear, unweighted: node 0 pulled by (3, 0)
+    np.testing.assert_allclose(oracle.attraction_general(pos, s, t), [[3, 0], [-3, 0]], rtol=1e-7)
+    # linlog: magnitude log(1 + d)
+    f = oracle.attraction_general(pos, s, t, linlog=True)
+    np.testing.assert_allclose(f, [[np.log(4.0), 0], [-np.log(4.0), 0]], rtol=1e-6)
+    # weights: w^delta
+    for delta, k in ((0.0, 1.0), (1.0, 2.0), (2.0, 4.0), (0.5, np.sqrt(2.0))):
+        f = oracle.attraction_general(pos, s, t, w=np.array([2.0]), delta=delta)
+        np.testing.assert_allclose(f[0], [3 * k, 0], rtol=1e-6)
+    f = oracle.attraction_general(pos, s, t, w=np.array([3.0]), delta=1.0, linlog=True)
+    np.testing.assert_allclose(f[0], [3 * np.log(4.0), 0], rtol=1e-6)
+    # coincident endpoints add nothing under linlog (the reference's 0-vector * FLT_MAX)
+    f = oracle.attraction_general(np.zeros((2, 2), np.float32), s, t, linlog=True)
+    assert np.all(f == 0)
+
+
+def test_attraction_general_sums_to_zero_and_matches_linear(oracle):
+    rng = np.random.default_rng(3)
+    n, m = 300, 1200
+    src = rng.integers(0, n, m)
+    dst = (src + 1 + rng.integers(0, n - 1, m)) % n
+    pos = rng.normal(size=(n, 2)).astype(np.float32) * 10
+    for kw in ({}, {"linlog": True}, {"w": rng.uniform(0.5, 2.0, m), "delta": 1.5}):
+        f = oracle.attraction_general(pos, src, dst, **kw).astype(np.float64)
+        assert np.abs(f.sum(0)).max() < 1e-3 * np.abs(f).max()
+    np.testing.assert_array_equal(oracle.attraction_general(pos, src, dst), oracle.attraction_coo(pos, src, dst))
