@@ -35,6 +35,7 @@ ABI_SYMBOLS = (
     "gfa2_set_graph_batch", "gfa2_get_batch_stats",
     "gfa2_autotune", "gfa2_get_launch_config", "gfa2_set_launch_config", "gfa2_tuned_config",
     "gfa2_set_edge_weights", "gfa2_set_graph_coo_weighted", "gfa2_exact_repulsion",
+    "gfa2_get_bbox", "gfa2_render",
 )
 
 
@@ -94,6 +95,15 @@ class LaunchConfig(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Raster(ctypes.Structure):
+    """Frame of the PNG picture (gfa2_raster)."""
+    _fields_ = [
+        ("x0", ctypes.c_float), ("y0", ctypes.c_float), ("sx", ctypes.c_float), ("sy", ctypes.c_float),
+        ("node_alpha", ctypes.c_float), ("edge_alpha", ctypes.c_float),
+        ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+    ]
+
+
 class GFA2Error(RuntimeError):
     def __init__(self, code: int, what: str):
         self.code = code
@@ -149,6 +159,8 @@ def lib() -> ctypes.CDLL:
     L.gfa2_set_edge_weights.argtypes = [_P, _I64, _P]
     L.gfa2_set_graph_coo_weighted.argtypes = [_P, _I32, _I64, _P, _P, _P, _P]
     L.gfa2_exact_repulsion.argtypes = [_P, _I32, _P]
+    L.gfa2_get_bbox.argtypes = [_P, _P]
+    L.gfa2_render.argtypes = [_P, ctypes.POINTER(Raster), _P]
     for name in ABI_SYMBOLS:
         fn = getattr(L, name)
         if name not in ("gfa2_default_params", "gfa2_strerror"):
@@ -328,6 +340,18 @@ class Engine:
         """All-pairs repulsion of the current positions (V x 2, float64)."""
         out = np.empty((self.n, 2), np.float64)
         _check(lib().gfa2_exact_repulsion(self.h, int(bool(fp64)), _ptr(out)), "gfa2_exact_repulsion")
+        return out
+
+    def bbox(self) -> np.ndarray:
+        """{min x, max x, min y, max y} of the device positions."""
+        out = np.zeros(4, np.float32)
+        _check(lib().gfa2_get_bbox(self.h, _ptr(out)), "gfa2_get_bbox")
+        return out
+
+    def render(self, frame: "Raster") -> np.ndarray:
+        """The PNG picture rasterised on the device: height x width grey bytes, top row first."""
+        out = np.zeros((frame.height, frame.width), np.uint8)
+        _check(lib().gfa2_render(self.h, ctypes.byref(frame), _ptr(out)), "gfa2_render")
         return out
 
     def autotune(self, iters: int = 10) -> dict:

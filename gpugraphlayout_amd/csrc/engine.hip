@@ -1095,6 +1095,14 @@ int gfa2_normalized_stress(gfa2_handle h, const float* xy, double* ns, double* a
 
 namespace gfa2 {
 
+bool engine_graph(gfa2_engine* h, const Bufs** b, int* device, hipStream_t* stream) {
+    if (!h || !h->has_graph || !h->has_pos) return false;
+    *b = &h->b;
+    *device = h->device;
+    *stream = h->stream;
+    return true;
+}
+
 bool engine_view(gfa2_engine* h, EngineView* v) {
     if (!h || !h->has_graph) return false;
     v->n = h->b.n;

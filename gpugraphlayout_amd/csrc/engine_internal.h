@@ -209,6 +209,8 @@ struct EngineView {
     hipStream_t stream;
 };
 bool engine_view(struct ::gfa2_engine* h, EngineView* v);
+// The buffers, device and stream of an engine holding a graph and positions (rasteriser).
+bool engine_graph(struct ::gfa2_engine* h, const Bufs** b, int* device, hipStream_t* stream);
 int engine_positions_written(struct ::gfa2_engine* h);
 
 // Walk grid sizes (partials count depends on them).

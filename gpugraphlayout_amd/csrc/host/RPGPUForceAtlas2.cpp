@@ -98,6 +98,18 @@ namespace RPGraph
 
     void GPUForceAtlas2::load_layout() { check(gfa2_set_positions(h, layout.data()), "gfa2_set_positions"); }
 
+    bool GPUForceAtlas2::writeToPNG(int image_w, int image_h, std::string path)
+    {
+        if (image_w <= 0 || image_h <= 0) return false;
+        float bb[4];
+        check(gfa2_get_bbox(h, bb), "gfa2_get_bbox");
+        const gfa2_raster f = png_frame(bb[0], bb[1], bb[2], bb[3], image_w, image_h, layout.graph.num_nodes(),
+                                        layout.graph.num_edges());
+        std::vector<unsigned char> gray((size_t)image_w * image_h);
+        check(gfa2_render(h, &f, gray.data()), "gfa2_render");
+        return write_png_gray(path, image_w, image_h, gray.data());
+    }
+
     void GPUForceAtlas2::use_graph_capture(bool on) { check(gfa2_set_graph_capture(h, on ? 1 : 0), "capture"); }
 
     gfa2_stats GPUForceAtlas2::stats()

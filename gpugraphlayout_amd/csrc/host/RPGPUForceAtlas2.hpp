@@ -35,6 +35,9 @@ namespace RPGraph
         void sync_layout() override;      // device positions -> layout
         void load_layout();               // layout -> device positions
         void use_graph_capture(bool on);
+        // The picture of GraphLayout::writeToPNG, rasterised on the device from the device
+        // positions (the same bytes as the CPU writer on the same positions).
+        bool writeToPNG(int image_w, int image_h, std::string path);
         gfa2_stats stats();
         gfa2_handle handle() { return h; }
 

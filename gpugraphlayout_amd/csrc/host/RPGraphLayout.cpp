@@ -108,41 +108,58 @@ namespace RPGraph
 
     namespace
     {
-        // Multiplicative ink buffer: a pixel covered with opacity a keeps (1-a) of its
-        // brightness; products commute, so drawing order does not matter.
-        struct Canvas
+        // Per-pixel hit counters for nodes and edges; y up, as the reference's canvas.
+        struct Counts
         {
             int w, h;
-            std::vector<float> keep;
-            Canvas(int w, int h) : w(w), h(h), keep((size_t)w * h, 1.0f) {}
-            void blend(int x, int y, float a)
+            std::vector<uint32_t> nodes, edges;
+            Counts(int w, int h) : w(w), h(h), nodes((size_t)w * h, 0), edges((size_t)w * h, 0) {}
+            static void hit(std::vector<uint32_t>& c, int w, int h, long x, long y)
             {
                 if (x < 0 || y < 0 || x >= w || y >= h) return;
-                keep[(size_t)(h - 1 - y) * w + x] *= (1.0f - a);   // y up, as the reference's canvas
+                c[(size_t)(h - 1 - y) * w + x] += 1;
             }
-            void disc(float cx, float cy, int r, float a)
+            void disc(float cx, float cy)
             {
-                const int x0 = (int)std::lround(cx), y0 = (int)std::lround(cy);
-                for (int dy = -r; dy <= r; ++dy)
-                    for (int dx = -r; dx <= r; ++dx)
-                        if (dx * dx + dy * dy <= r * r) blend(x0 + dx, y0 + dy, a);
+                const long x0 = std::lround(cx), y0 = std::lround(cy);
+                for (int dy = -3; dy <= 3; ++dy)
+                    for (int dx = -3; dx <= 3; ++dx)
+                        if (dx * dx + dy * dy <= 9) hit(nodes, w, h, x0 + dx, y0 + dy);
             }
-            void line(float x0, float y0, float x1, float y1, float a)
+            // ceil(max(|dx|, |dy|)) + 1 evenly spaced samples (positions formed in double),
+            // each rounded to a pixel.
+            void line(float x0, float y0, float x1, float y1)
             {
-                const float len = std::fmax(std::fabs(x1 - x0), std::fabs(y1 - y0));
+                const float ddx = x1 - x0, ddy = y1 - y0;
+                const float len = std::fmax(std::fabs(ddx), std::fabs(ddy));
                 const int steps = (int)std::ceil(len);
                 if (steps == 0)
                 {
-                    blend((int)std::lround(x0), (int)std::lround(y0), a);
+                    hit(edges, w, h, std::lround(x0), std::lround(y0));
                     return;
                 }
                 for (int k = 0; k <= steps; ++k)
                 {
-                    const float t = (float)k / steps;
-                    blend((int)std::lround(x0 + t * (x1 - x0)), (int)std::lround(y0 + t * (y1 - y0)), a);
+                    const double t = (double)k / (double)steps;
+                    const float sx = (float)((double)x0 + t * (double)ddx);
+                    const float sy = (float)((double)y0 + t * (double)ddy);
+                    hit(edges, w, h, std::lround(sx), std::lround(sy));
                 }
             }
         };
+
+        // b^k by repeated squaring (the device rasteriser uses the same sequence).
+        float powi(float b, uint32_t k)
+        {
+            float r = 1.0f;
+            while (k)
+            {
+                if (k & 1u) r = r * b;
+                b = b * b;
+                k >>= 1;
+            }
+            return r;
+        }
 
         void put32(std::vector<unsigned char>& v, uint32_t x)
         {
@@ -167,6 +184,40 @@ namespace RPGraph
         }
     }
 
+    gfa2_raster png_frame(float minx, float maxx, float miny, float maxy, int image_w, int image_h, nid_t nodes,
+                          nid_t edges)
+    {
+        gfa2_raster f;
+        const float xr = maxx - minx, yr = maxy - miny;
+        const float cx = minx + xr / 2.0, cy = miny + yr / 2.0;
+        f.x0 = cx - xr / 2.0;
+        f.y0 = cy - yr / 2.0;
+        f.sx = xr > 0 ? image_w / xr : 1.0f;
+        f.sy = yr > 0 ? image_h / yr : 1.0f;
+        f.node_alpha = std::fmin(1.0f, (float)(10000.0 / std::max<nid_t>(nodes, 1)));
+        f.edge_alpha = std::fmin(1.0f, (float)(100000.0 / std::max<nid_t>(edges, 1)));
+        f.width = image_w;
+        f.height = image_h;
+        return f;
+    }
+
+    void render_gray(const gfa2_raster& f, const UGraph& graph, const float* xy, std::vector<unsigned char>& gray)
+    {
+        Counts cv(f.width, f.height);
+        const nid_t V = graph.num_nodes();
+        for (nid_t n1 = 0; n1 < V; ++n1)
+        {
+            const float ax = (xy[2 * n1] - f.x0) * f.sx, ay = (xy[2 * n1 + 1] - f.y0) * f.sy;
+            cv.disc(ax, ay);
+            for (nid_t n2 : graph.neighbors_with_geq_id(n1))
+                cv.line(ax, ay, (xy[2 * n2] - f.x0) * f.sx, (xy[2 * n2 + 1] - f.y0) * f.sy);
+        }
+        const float keep_n = 1.0f - f.node_alpha, keep_e = 1.0f - f.edge_alpha;
+        gray.resize((size_t)f.width * f.height);
+        for (size_t p = 0; p < gray.size(); ++p)
+            gray[p] = (unsigned char)std::lround(255.0f * (powi(keep_n, cv.nodes[p]) * powi(keep_e, cv.edges[p])));
+    }
+
     // PNG of the layout: the layout's bounding box fitted to the image, white background,
     // nodes as discs of radius 3 with opacity 10000/V, edges as lines with opacity
     // 100000/E, both black and blended (src/RPGraphLayout.cpp:170-207; the reference draws
@@ -175,21 +226,15 @@ namespace RPGraph
     bool GraphLayout::writeToPNG(const int image_w, const int image_h, std::string path)
     {
         if (image_w <= 0 || image_h <= 0) return false;
-        const nid_t V = graph.num_nodes();
-        const float xr = getXRange(), yr = getYRange();
-        const Coordinate c = getCenter();
-        const float x0 = c.x - xr / 2.0, y0 = c.y - yr / 2.0;
-        const float sx = xr > 0 ? image_w / xr : 1.0f, sy = yr > 0 ? image_h / yr : 1.0f;
-        const float node_a = std::fmin(1.0f, (float)(10000.0 / std::max<nid_t>(V, 1)));
-        const float edge_a = std::fmin(1.0f, (float)(100000.0 / std::max<nid_t>(graph.num_edges(), 1)));
-        Canvas cv(image_w, image_h);
-        for (nid_t n1 = 0; n1 < V; ++n1)
-        {
-            const float ax = (getX(n1) - x0) * sx, ay = (getY(n1) - y0) * sy;
-            cv.disc(ax, ay, 3, node_a);
-            for (nid_t n2 : graph.neighbors_with_geq_id(n1))
-                cv.line(ax, ay, (getX(n2) - x0) * sx, (getY(n2) - y0) * sy, edge_a);
-        }
+        const gfa2_raster f = png_frame(minX(), maxX(), minY(), maxY(), image_w, image_h, graph.num_nodes(),
+                                        graph.num_edges());
+        std::vector<unsigned char> gray;
+        render_gray(f, graph, xy.data(), gray);
+        return write_png_gray(path, image_w, image_h, gray.data());
+    }
+
+    bool write_png_gray(const std::string& path, int image_w, int image_h, const unsigned char* gray)
+    {
         std::vector<unsigned char> raw;
         raw.reserve((size_t)image_h * (3 * image_w + 1));
         for (int y = 0; y < image_h; ++y)
@@ -197,7 +242,7 @@ namespace RPGraph
             raw.push_back(0);
             for (int x = 0; x < image_w; ++x)
             {
-                const unsigned char v = (unsigned char)std::lround(255.0f * cv.keep[(size_t)y * image_w + x]);
+                const unsigned char v = gray[(size_t)y * image_w + x];
                 raw.push_back(v);
                 raw.push_back(v);
                 raw.push_back(v);

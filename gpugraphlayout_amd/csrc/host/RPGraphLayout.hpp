@@ -9,6 +9,7 @@
 #include <vector>
 #include "RPCommon.hpp"
 #include "RPGraph.hpp"
+#include "gfa2.h"
 
 namespace RPGraph
 {
@@ -50,6 +51,15 @@ namespace RPGraph
     private:
         std::vector<float> xy;
     };
+
+    // The picture of writeToPNG, shared by the CPU writer and the device rasteriser
+    // (gfa2_render): the frame fitting the bounding box to the image (arithmetic of
+    // src/RPGraphLayout.cpp:170-185), the CPU rasteriser producing the same bytes as the
+    // device one, and the PNG encoder (8-bit grey written as RGB, zlib-deflated).
+    gfa2_raster png_frame(float minx, float maxx, float miny, float maxy, int image_w, int image_h, nid_t nodes,
+                          nid_t edges);
+    void render_gray(const gfa2_raster& f, const UGraph& graph, const float* xy, std::vector<unsigned char>& gray);
+    bool write_png_gray(const std::string& path, int image_w, int image_h, const unsigned char* gray);
 }
 
 #endif

@@ -80,6 +80,29 @@ int gfa2host_write_layout(gfa2host_graph_t h, const float* xy, const char* path,
 }
 
 void gfa2host_free(gfa2host_graph_t h) { delete h; }
+
+void gfa2host_png_frame(float minx, float maxx, float miny, float maxy, int32_t image_w, int32_t image_h,
+                        int64_t nodes, int64_t edges, gfa2_raster* out)
+{
+    if (out)
+        *out = RPGraph::png_frame(minx, maxx, miny, maxy, image_w, image_h, (RPGraph::nid_t)nodes,
+                                  (RPGraph::nid_t)edges);
+}
+
+int gfa2host_render_gray(gfa2host_graph_t h, const float* xy, const gfa2_raster* f, uint8_t* gray)
+{
+    if (!h || !xy || !f || !gray || f->width <= 0 || f->height <= 0) return -1;
+    std::vector<unsigned char> g;
+    RPGraph::render_gray(*f, h->g, xy, g);
+    memcpy(gray, g.data(), g.size());
+    return 0;
+}
+
+int gfa2host_write_png_gray(const char* path, int32_t image_w, int32_t image_h, const uint8_t* gray)
+{
+    if (!path || !gray || image_w <= 0 || image_h <= 0) return -1;
+    return RPGraph::write_png_gray(path, image_w, image_h, gray) ? 0 : -1;
+}
 }
 
 namespace

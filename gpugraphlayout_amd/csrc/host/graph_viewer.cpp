@@ -117,7 +117,7 @@ int main(int argc, const char** argv)
                 fflush(stdout);
                 fa2.sync_layout();
                 bool ok = true;
-                if (out_format == "png") ok = layout.writeToPNG(image_w, image_h, file);
+                if (out_format == "png") ok = fa2.writeToPNG(image_w, image_h, file);   // rasterised on the GPU
                 else if (out_format == "csv") ok = layout.writeToCSV(file);
                 else ok = layout.writeToBin(file);
                 if (!ok) return EXIT_FAILURE;

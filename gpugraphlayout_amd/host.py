@@ -31,6 +31,12 @@ def lib():
         L.gfa2host_rmat.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_uint64, _P, _P]
         L.gfa2host_rmat.restype = ctypes.c_int64
+        from . import Raster
+        L.gfa2host_png_frame.argtypes = [ctypes.c_float] * 4 + [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                                                  ctypes.c_int64, ctypes.POINTER(Raster)]
+        L.gfa2host_png_frame.restype = None
+        L.gfa2host_render_gray.argtypes = [_P, _P, ctypes.POINTER(Raster), _P]
+        L.gfa2host_write_png_gray.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, _P]
         _lib = L
     return _lib
 
@@ -88,3 +94,27 @@ class HostGraph:
 
     def __del__(self):
         self.close()
+
+
+def png_frame(bbox, width: int, height: int, nodes: int, edges: int):
+    """The frame fitting bbox = (min x, max x, min y, max y) to a width x height image."""
+    from . import Raster
+    f = Raster()
+    lib().gfa2host_png_frame(float(bbox[0]), float(bbox[1]), float(bbox[2]), float(bbox[3]), int(width), int(height),
+                             int(nodes), int(edges), ctypes.byref(f))
+    return f
+
+
+def render_gray(hg: "HostGraph", xy, frame) -> np.ndarray:
+    """The CPU rasteriser of the PNG picture: height x width grey bytes, top row first."""
+    xy = np.ascontiguousarray(xy, np.float32)
+    out = np.zeros((frame.height, frame.width), np.uint8)
+    if lib().gfa2host_render_gray(hg.h, _p(xy), ctypes.byref(frame), _p(out)) != 0:
+        raise ValueError("render_gray")
+    return out
+
+
+def write_png_gray(path: str, gray: np.ndarray):
+    gray = np.ascontiguousarray(gray, np.uint8)
+    if lib().gfa2host_write_png_gray(path.encode(), gray.shape[1], gray.shape[0], _p(gray)) != 0:
+        raise OSError(path)

@@ -193,6 +193,22 @@ int gfa2_set_graph_coo_weighted(gfa2_handle h, int32_t n, int64_t m, const int32
  * ascending order; xy gets V (x, y) pairs. Does not advance the layout. */
 int gfa2_exact_repulsion(gfa2_handle h, int32_t fp64, double* xy);
 
+/* Rasteriser on the device for GraphLayout::writeToPNG (src/RPGraphLayout.cpp:170-207). The
+ * frame maps layout point (x, y) to pixel ((x - x0) * sx, (y - y0) * sy), y up; the host layer
+ * fills it from gfa2_get_bbox as its CPU writer does. Nodes are discs of radius 3, edges lines
+ * from the smaller to the larger node id; a pixel hit by k nodes and l edges gets brightness
+ * round(255 (1 - node_alpha)^k (1 - edge_alpha)^l). gray: width*height bytes, top row first.
+ * Single-graph handles only (GFA2_ERR_UNSUPPORTED for a batch). */
+typedef struct gfa2_raster {
+    float x0, y0;
+    float sx, sy;
+    float node_alpha, edge_alpha;
+    int32_t width, height;
+} gfa2_raster;
+/* Exact bounding box of the device positions: {min x, max x, min y, max y}. */
+int gfa2_get_bbox(gfa2_handle h, float* out4);
+int gfa2_render(gfa2_handle h, const gfa2_raster* r, uint8_t* gray);
+
 /* Graph capture of the whole iteration (hipGraph), replayed per iteration. */
 int gfa2_set_graph_capture(gfa2_handle h, int32_t on);
 

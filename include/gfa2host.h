@@ -11,6 +11,7 @@
 #ifndef GFA2HOST_H
 #define GFA2HOST_H
 #include <stdint.h>
+#include "gfa2.h"
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -31,6 +32,14 @@ int gfa2host_save_binary(gfa2host_graph_t g, const char* path);
 int gfa2host_write_layout(gfa2host_graph_t g, const float* xy, const char* path, const char* fmt, int32_t image_w,
                           int32_t image_h);
 void gfa2host_free(gfa2host_graph_t g);
+
+/* The PNG picture (src/RPGraphLayout.cpp:170-207): the frame fitting a bounding box to the
+ * image, the CPU rasteriser (the same bytes gfa2_render produces on the device for the same
+ * positions) and the PNG encoder for a width*height grey image (top row first). */
+void gfa2host_png_frame(float minx, float maxx, float miny, float maxy, int32_t image_w, int32_t image_h,
+                        int64_t nodes, int64_t edges, gfa2_raster* out);
+int gfa2host_render_gray(gfa2host_graph_t g, const float* xy, const gfa2_raster* f, uint8_t* gray);
+int gfa2host_write_png_gray(const char* path, int32_t image_w, int32_t image_h, const uint8_t* gray);
 
 /* Barabasi-Albert preferential attachment, deterministic per seed (splitmix64): a star
  * K_{1,m} on nodes 0..m, then each node v = m+1..n-1 attaches to m distinct earlier nodes

@@ -85,3 +85,78 @@ def test_png_writer(tmp_path):
     px = raw[:, 1:]
     assert px.max() == 255                 # white background remains
     assert 0.01 < (px < 255).mean() < 0.9   # ink from nodes and edges
+
+
+def _render_numpy(g, xy, f):
+    """The PNG picture restated with numpy float32 scalars: frame mapping, discs of radius 3,
+    ceil(max(|dx|,|dy|))+1 rounded samples per edge (smaller id first; sample positions in
+    float64), per-pixel hit counts,
+    brightness round(255 (1-a_n)^k (1-a_e)^l) with powers by repeated squaring."""
+    F = np.float32
+    W, H = f.width, f.height
+    cn = np.zeros((H, W), np.int64)
+    ce = np.zeros((H, W), np.int64)
+
+    def rnd(v):  # round half away from zero, as lround
+        return int(np.floor(abs(float(v)) + 0.5)) * (1 if v >= 0 else -1)
+
+    def hit(c, x, y):
+        if 0 <= x < W and 0 <= y < H:
+            c[H - 1 - y, x] += 1
+
+    px = [(F(F(xy[i, 0]) - F(f.x0)) * F(f.sx), F(F(xy[i, 1]) - F(f.y0)) * F(f.sy)) for i in range(g.n)]
+    for i in range(g.n):
+        cx, cy = rnd(px[i][0]), rnd(px[i][1])
+        for dy in range(-3, 4):
+            for dx in range(-3, 4):
+                if dx * dx + dy * dy <= 9:
+                    hit(cn, cx + dx, cy + dy)
+    for a, b in zip(g.src, g.dst):
+        i, j = min(a, b), max(a, b)
+        x0, y0 = px[i]
+        x1, y1 = px[j]
+        ddx, ddy = F(x1 - x0), F(y1 - y0)
+        steps = int(np.ceil(max(abs(ddx), abs(ddy))))
+        if steps == 0:
+            hit(ce, rnd(x0), rnd(y0))
+            continue
+        for k in range(steps + 1):
+            t = np.float64(k) / np.float64(steps)
+            hit(ce, rnd(F(np.float64(x0) + t * np.float64(ddx))), rnd(F(np.float64(y0) + t * np.float64(ddy))))
+
+    def powi(b, k):
+        r = F(1.0)
+        b = F(b)
+        while k:
+            if k & 1:
+                r = F(r * b)
+            b = F(b * b)
+            k >>= 1
+        return r
+
+    kn, ke = F(F(1.0) - F(f.node_alpha)), F(F(1.0) - F(f.edge_alpha))
+    out = np.zeros((H, W), np.uint8)
+    for y in range(H):
+        for x in range(W):
+            out[y, x] = rnd(F(F(255.0) * F(powi(kn, cn[y, x]) * powi(ke, ce[y, x]))))
+    return out
+
+
+def test_cpu_rasteriser_matches_numpy_restatement():
+    from gpugraphlayout_amd.host import png_frame, render_gray
+    s, t = graphs.barabasi_albert(120, 2, 4)
+    hg = HostGraph.from_edges(s, t)
+    g = hg.arrays()
+    xy = graphs.uniform_positions(g.n, 30, 8)
+    bb = (xy[:, 0].min(), xy[:, 0].max(), xy[:, 1].min(), xy[:, 1].max())
+    f = png_frame(bb, 64, 40, g.n, g.m)
+    # the frame: the bounding box spans the image
+    assert abs(float(f.sx) - 64 / (bb[1] - bb[0])) < 1e-4 * float(f.sx)
+    assert f.node_alpha == 1.0 and f.edge_alpha == 1.0   # 10000/V and 100000/E exceed 1 here
+    got = render_gray(hg, xy, f)
+    np.testing.assert_array_equal(got, _render_numpy(g, xy, f))
+    # partial opacity: a frame with alphas below one
+    f.node_alpha, f.edge_alpha = 0.3, 0.05
+    got = render_gray(hg, xy, f)
+    np.testing.assert_array_equal(got, _render_numpy(g, xy, f))
+    assert got.max() == 255 and got.min() < 100
