@@ -193,30 +193,36 @@ __device__ __forceinline__ int xcd_block() {
 // advance of the record index (1, or the subtree size of an accepted cell). A step that is
 // not live contributes nothing: fma(d, 0, a) == a, since a is never -0 here.
 template <int G>
-__device__ __forceinline__ int walk_step(const float4 c, bool live, bool active, float px, float py, float kmi,
-                                         float base, float epssq, float& ax, float& ay, unsigned int& inter) {
+__device__ __forceinline__ int walk_step(const float4 c, bool live, unsigned long long act, float px, float py,
+                                         float kmi, float base, float epssq, float& ax, float& ay,
+                                         unsigned int& inter) {
     const unsigned int info = __float_as_uint(c.w);
     const float dx = px - c.x;
     const float dy = py - c.y;
     const float d2 = dx * dx + dy * dy + epssq;
-    const bool body = info == 0xffffffffu;
     const float thr = ldexpf(base, -2 * (int)(info & 31u)) + epssq;
-    const unsigned long long fails = __builtin_amdgcn_ballot_w64(live && active && !body && !(d2 >= thr));
-    // The group's vote as a lane mask built with scalar operations (each half-wave is one
-    // group when G == 32), read back per lane without touching vector registers.
+    // Lane masks from plain compares (each lands in a scalar register pair), combined with
+    // scalar operations: bodies, lanes closer than the threshold, live lanes.
+    const unsigned long long body = __builtin_amdgcn_ballot_w64(info == 0xffffffffu);
+    const unsigned long long near = __builtin_amdgcn_ballot_w64(!(d2 >= thr));
+    const unsigned long long livem = __builtin_amdgcn_ballot_w64(live);
+    const unsigned long long fails = near & ~body & act & livem;
+    // The group's vote (each half-wave is one group when G == 32).
     unsigned long long okm;
     if constexpr (G == 32)
-        okm = ((unsigned int)fails == 0u ? 0xffffffffull : 0ull) |
-              ((unsigned int)(fails >> 32) == 0u ? 0xffffffff00000000ull : 0ull);
+        okm = (((unsigned int)fails == 0u) ? 0x00000000ffffffffull : 0ull) |
+              (((fails & 0xffffffff00000000ull) == 0ull) ? 0xffffffff00000000ull : 0ull);
     else
         okm = fails == 0ull ? ~0ull : 0ull;
-    const bool pass = __builtin_amdgcn_inverse_ballot_w64(okm);
-    const bool accept = live & (body | pass);   // no short circuit: keeps the step branch-free
+    const unsigned long long accm = (body | okm) & livem;
+    const bool accept = __builtin_amdgcn_inverse_ballot_w64(accm);
+    const bool jump = __builtin_amdgcn_inverse_ballot_w64(accm & ~body);
+    inter += __builtin_amdgcn_inverse_ballot_w64(accm & act) ? 1u : 0u;
+    // Branch-free: a rejected cell adds fma(d, 0, a) == a (a is never -0 here).
     const float w = accept ? kmi * c.z * __builtin_amdgcn_rcpf(d2) : 0.0f;
     ax = fmaf(dx, w, ax);
     ay = fmaf(dy, w, ay);
-    inter += (active && accept) ? 1u : 0u;
-    return (accept && !body) ? max((int)(info >> kInfoLevelBits), 1) : 1;
+    return jump ? max((int)(info >> kInfoLevelBits), 1) : 1;
 }
 
 // Record fetch modes (none changes a result bit: same records, same order, same arithmetic):
@@ -262,6 +268,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     // Every lane of a group walks, active or not, so the group's record index stays uniform.
     const int gbase = (G == 32) ? (lane & 32) : 0;
     if (group_ok) {
+        const unsigned long long act = __builtin_amdgcn_ballot_w64(active);
         const float two_r = seg_root[g].z * 2;
         const float base = two_r * two_r * itolsq;
         int r = rec_begin[g] + 1;
@@ -281,20 +288,20 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
                 c.y = __shfl(wnd.y, src, 64);
                 c.z = __shfl(wnd.z, src, 64);
                 c.w = __shfl(wnd.w, src, 64);
-                r += walk_step<G>(c, true, active, px, py, kmi, base, epssq, ax, ay, inter);
+                r += walk_step<G>(c, true, act, px, py, kmi, base, epssq, ax, ay, inter);
             }
         } else if constexpr (MODE == 2) {
             while (r < end) {
                 const bool has1 = r + 1 < end;
                 const float4 c0 = rec[r];
                 const float4 c1 = rec[has1 ? r + 1 : r];
-                const int a0 = walk_step<G>(c0, true, active, px, py, kmi, base, epssq, ax, ay, inter);
+                const int a0 = walk_step<G>(c0, true, act, px, py, kmi, base, epssq, ax, ay, inter);
                 const bool live1 = has1 && a0 == 1;
-                const int a1 = walk_step<G>(c1, live1, active, px, py, kmi, base, epssq, ax, ay, inter);
+                const int a1 = walk_step<G>(c1, live1, act, px, py, kmi, base, epssq, ax, ay, inter);
                 r += live1 ? 1 + a1 : a0;
             }
         } else {
-            while (r < end) r += walk_step<G>(rec[r], true, active, px, py, kmi, base, epssq, ax, ay, inter);
+            while (r < end) r += walk_step<G>(rec[r], true, act, px, py, kmi, base, epssq, ax, ay, inter);
         }
     }
     group_finish<G>(active, group_ok, gl, gidx, id, mi, ax, ay, inter, fga, fprev, f, rep, part);
