@@ -443,6 +443,14 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.sort.vals[1], n);
     GFA2_ALLOC(b.sort.hist, (size_t)256 * sort_tiles);
     GFA2_ALLOC(b.sort.rowtot, 256);
+    GFA2_ALLOC(b.sort.digit_hist, 8 * 256);
+    GFA2_ALLOC(b.sort.status, (size_t)256 * sort_tiles);
+    GFA2_ALLOC(b.sort.ticket, 1);
+    GFA2_ALLOC(b.sort.err, 1);
+    // look-back state starts empty: no published tile, ticket 0, no error
+    GFA2_TRY(hipMemsetAsync(b.sort.status, 0, sizeof(unsigned int) * 256 * (size_t)sort_tiles, h->stream));
+    GFA2_TRY(hipMemsetAsync(b.sort.ticket, 0, sizeof(unsigned long long), h->stream));
+    GFA2_TRY(hipMemsetAsync(b.sort.err, 0, sizeof(int), h->stream));
     GFA2_ALLOC(b.lcp, n);
     GFA2_ALLOC(b.blocksum, tree_tiles);
     GFA2_ALLOC(b.S, (size_t)n + 1);
@@ -738,6 +746,12 @@ int gfa2_get_stats(gfa2_handle h, gfa2_stats* s) {
     s->root_y = st.rooty;
     s->radius = st.radius;
     s->error_flags = st.err;
+    if (h->b.sort.err) {
+        int sort_err = 0;
+        GFA2_TRY(hipMemcpyAsync(&sort_err, h->b.sort.err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        GFA2_TRY(hipStreamSynchronize(h->stream));
+        if (sort_err) s->error_flags |= kErrSortWait;
+    }
     return GFA2_OK;
 }
 

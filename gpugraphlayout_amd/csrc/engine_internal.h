@@ -34,6 +34,7 @@ constexpr int kPartBlocks = 1024;       // bounding-box partials (displacement g
 
 // Error flags raised on the device, read back at synchronisation points.
 constexpr int kErrTreeOverflow = 1;
+constexpr int kErrSortWait = 2;         // a sort look-back wait ran out (results invalid)
 
 // Per-instance scalar state, resident in device memory.
 struct DevState {
@@ -70,8 +71,13 @@ struct WalkPart {
 struct SortBuffers {
     unsigned long long* keys[2];
     unsigned int* vals[2];
-    unsigned int* hist;      // 256 x tiles
+    unsigned int* hist;      // 256 x tiles (three-launch passes)
     unsigned int* rowtot;    // 256
+    bool onesweep = true;    // one launch per pass with decoupled look-back
+    unsigned int* digit_hist = nullptr;   // 8 x 256: digit totals of every pass
+    unsigned int* status = nullptr;       // tiles x 256: published tile counts / prefixes
+    unsigned long long* ticket = nullptr; // tile tickets, never reset while the tile count holds
+    int* err = nullptr;                   // set if a look-back wait ran out (never expected)
 };
 
 // FA2 scalar parameters (src/RPForceAtlas2.cpp:28-55 defaults).

@@ -1,13 +1,20 @@
 // LSD radix sort of (u64 key, u32 value) pairs for gfx950.
 //
-// Each 8-bit pass is three launches:
-//   upsweep   per-tile digit histograms (LDS atomics), stored digit-major
-//   rowscan   one wave per digit: exclusive prefix of that digit over tiles + total
-//   scatter   per tile: digit start = scan of the 256 totals (done redundantly in
-//             LDS), stable rank within the tile from wave64 ballot matching, then the
-//             scatter. Keys of one digit from one tile land contiguously.
-// Every cross-tile dependency goes through a launch boundary; no inter-workgroup
-// waiting, no global atomics. Stable, hence LSD-correct.
+// Up to 64 tiles (131,072 pairs): one launch per 8-bit pass ("onesweep"), after one histogram
+// launch for all passes.
+//   k_sort_hist  digit histograms of every pass at once (order-independent), LDS then global
+//                atomics; the table is cleared by k_sort_zero first
+//   k_onesweep   per tile (2048 pairs): tiles are taken in the order of a global ticket, the
+//                tile's digit counts are published, the exclusive prefix of each digit over the
+//                earlier tiles is found by looking back over their published counts (decoupled
+//                look-back), then the stable rank within the tile (wave64 ballot matching) gives
+//                each pair's destination. A tile only ever waits for tiles with smaller tickets,
+//                which started before it and publish their counts before they wait themselves,
+//                so the waits end; every wait is also bounded and raises an error flag.
+// Larger inputs (or SortBuffers::onesweep = false): three launches per pass (per-tile histograms,
+// a scan over tiles per digit, scatter), with every cross-tile dependency through a launch
+// boundary.
+// Both are stable (hence LSD-correct) and give the same result.
 #include "engine_internal.h"
 
 namespace gfa2 {
@@ -134,18 +141,181 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long lo
     }
 }
 
+// ---- onesweep -------------------------------------------------------------------------------
+constexpr unsigned int kStatAgg = 1u << 30;      // the tile's own digit count is published
+constexpr unsigned int kStatPre = 2u << 30;      // the inclusive prefix over tiles is published
+constexpr unsigned int kStatCount = (1u << 27) - 1;
+constexpr int kOnesweepMaxTiles = 64;
+constexpr int kMaxSpins = 1 << 20;   // per thread and look-back: ~1 s at worst, then an error
+
+__global__ void k_sort_zero(unsigned int* __restrict__ digit_hist, int npasses) {
+    for (int i = threadIdx.x; i < npasses * 256; i += blockDim.x) digit_hist[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_sort_hist(const unsigned long long* __restrict__ keys, int n, int begin_bit,
+                                                   int npasses, unsigned int* __restrict__ digit_hist) {
+    __shared__ unsigned int h[8][256];
+    for (int p = 0; p < npasses; ++p) h[p][threadIdx.x] = 0u;
+    __syncthreads();
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+        const unsigned long long k = keys[e];
+        for (int p = 0; p < npasses; ++p) atomicAdd(&h[p][(unsigned)(k >> (begin_bit + 8 * p)) & 255u], 1u);
+    }
+    __syncthreads();
+    for (int p = 0; p < npasses; ++p) {
+        const unsigned int c = h[p][threadIdx.x];
+        if (c) atomicAdd(&digit_hist[p * 256 + threadIdx.x], c);
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long long* __restrict__ kin,
+                                                           const unsigned int* __restrict__ vin,
+                                                           unsigned long long* __restrict__ kout,
+                                                           unsigned int* __restrict__ vout, int n, int shift,
+                                                           const unsigned int* __restrict__ digit_hist,
+                                                           unsigned int* __restrict__ status,
+                                                           unsigned long long* __restrict__ ticket, int tiles,
+                                                           int* __restrict__ err) {
+    __shared__ unsigned int dstart[256];
+    __shared__ unsigned int wcnt[kWaves][256];
+    __shared__ unsigned int scan_tmp[256];
+    __shared__ unsigned int tcount[256];
+    __shared__ int s_tile;
+    __shared__ unsigned int s_epoch;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    if (tid == 0) {
+        const unsigned long long t = atomicAdd(ticket, 1ull);
+        s_tile = (int)(t % (unsigned long long)tiles);
+        s_epoch = (unsigned int)((t / (unsigned long long)tiles) & 7ull);
+    }
+    tcount[tid] = 0u;
+    // exclusive scan of this pass's digit totals (Hillis-Steele in LDS)
+    const unsigned int tot = digit_hist[tid];
+    scan_tmp[tid] = tot;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        unsigned int v = tid >= o ? scan_tmp[tid - o] : 0u;
+        __syncthreads();
+        scan_tmp[tid] += v;
+        __syncthreads();
+    }
+    const unsigned int gstart = scan_tmp[tid] - tot;
+    const int tile = s_tile;
+    const unsigned int epoch = s_epoch << 27;
+    // the tile's pairs, held in registers, and its digit counts
+    const int base = tile * kSortTile;
+    unsigned long long key[kSortItems];
+    unsigned int val[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int e = base + r * kSortThreads + tid;
+        key[r] = e < n ? kin[e] : 0ull;
+        val[r] = e < n ? vin[e] : 0u;
+        if (e < n) atomicAdd(&tcount[(unsigned)(key[r] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    // publish, look back, publish the inclusive prefix (one digit per thread)
+    unsigned int* my = status + (size_t)tile * 256 + tid;
+    const unsigned int cnt = tcount[tid];
+    unsigned int excl = 0u;
+    if (tile == 0) {
+        __hip_atomic_store(my, kStatPre | epoch | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(my, kStatAgg | epoch | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int spins = 0;                       // bounded over the whole look-back
+        for (int t = tile - 1; t >= 0; --t) {
+            const unsigned int* src = status + (size_t)t * 256 + tid;
+            unsigned int v = 0u;
+            for (;;) {
+                v = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((v >> 30) != 0u && (v & (7u << 27)) == epoch) break;
+                if (++spins >= kMaxSpins) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (spins >= kMaxSpins) {
+                atomicOr(err, 1);
+                break;
+            }
+            excl += v & kStatCount;
+            if ((v >> 30) == 2u) break;
+        }
+        __hip_atomic_store(my, kStatPre | epoch | ((excl + cnt) & kStatCount), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    dstart[tid] = gstart + excl;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    // stable ranks within the tile, row by row (the fallback's scatter, from registers)
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int e = base + r * kSortThreads + tid;
+        const bool valid = e < n;
+        const unsigned int d = (unsigned)(key[r] >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            unsigned long long m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const unsigned int rank = (unsigned)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][d] = (unsigned)__popcll(peers);
+        __syncthreads();
+        {
+            unsigned int run = dstart[tid];
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                unsigned int c = wcnt[w][tid];
+                wcnt[w][tid] = run;
+                run += c;
+            }
+            dstart[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const unsigned int dst = wcnt[wave][d] + rank;
+            kout[dst] = key[r];
+            vout[dst] = val[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s) {
     if (n <= 1) return;
     const int tiles = (n + kSortTile - 1) / kSortTile;
     int cur = 0;
-    for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        k_upsweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], n, shift, buf.hist, tiles);
-        k_rowscan<<<64, 256, 0, s>>>(buf.hist, tiles, buf.rowtot);
-        k_scatter<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], buf.vals[cur], buf.keys[cur ^ 1],
-                                                 buf.vals[cur ^ 1], n, shift, buf.hist, tiles, buf.rowtot);
-        cur ^= 1;
+    // One launch per pass pays off while the look-back chains stay short; with hundreds of
+    // tiles resident at once the chains grow with the tile count and three launches per pass win.
+    if (buf.onesweep && tiles <= kOnesweepMaxTiles) {
+        const int npasses = (end_bit - begin_bit + 7) / 8;
+        k_sort_zero<<<1, 256, 0, s>>>(buf.digit_hist, npasses);
+        int hb = (n + 255) / 256;
+        if (hb > 256) hb = 256;
+        k_sort_hist<<<hb, 256, 0, s>>>(buf.keys[0], n, begin_bit, npasses, buf.digit_hist);
+        for (int p = 0; p < npasses; ++p) {
+            k_onesweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], buf.vals[cur], buf.keys[cur ^ 1],
+                                                      buf.vals[cur ^ 1], n, begin_bit + 8 * p,
+                                                      buf.digit_hist + 256 * p, buf.status, buf.ticket, tiles,
+                                                      buf.err);
+            cur ^= 1;
+        }
+    } else {
+        for (int shift = begin_bit; shift < end_bit; shift += 8) {
+            k_upsweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], n, shift, buf.hist, tiles);
+            k_rowscan<<<64, 256, 0, s>>>(buf.hist, tiles, buf.rowtot);
+            k_scatter<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], buf.vals[cur], buf.keys[cur ^ 1],
+                                                     buf.vals[cur ^ 1], n, shift, buf.hist, tiles, buf.rowtot);
+            cur ^= 1;
+        }
     }
     if (cur != 0) {
         hipMemcpyAsync(buf.keys[0], buf.keys[1], sizeof(unsigned long long) * n, hipMemcpyDeviceToDevice, s);

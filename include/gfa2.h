@@ -69,7 +69,7 @@ typedef struct gfa2_stats {
     int32_t cells;              /* quadtree cells of the last tree */
     int32_t max_level;          /* deepest cell level (root 0); max body depth = max_level + 1 */
     float root_x, root_y, radius;
-    int32_t error_flags;
+    int32_t error_flags;        /* bit 0: tree capacity exceeded; bit 1: a sort wait ran out */
 } gfa2_stats;
 
 /* Force components for gfa2_get_forces. */
