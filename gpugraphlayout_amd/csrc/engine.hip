@@ -373,7 +373,8 @@ int build_groups(gfa2_engine* h) {
 // uploads the graph. node_off: nseg+1 node offsets (nseg = 1 for a single graph).
 int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32_t* col, const float* mass,
                    int nseg, const int32_t* node_off) {
-    if (n < 1 || nseg < 1 || nseg > n) return GFA2_ERR_INVALID_GRAPH;
+    // bodies and cells share the 27-bit record index space: up to 2^26 bodies
+    if (n < 1 || n > (1 << 26) || nseg < 1 || nseg > n) return GFA2_ERR_INVALID_GRAPH;
     GFA2_TRY(hipSetDevice(h->device));
     if (rowptr[0] != 0) return GFA2_ERR_INVALID_GRAPH;
     if (node_off[0] != 0 || node_off[nseg] != n) return GFA2_ERR_INVALID_GRAPH;
@@ -612,15 +613,26 @@ int gfa2_step(gfa2_handle h, int32_t iters) {
     if (!h->has_pos) return GFA2_ERR_INVALID_ARG;
     GFA2_TRY(hipSetDevice(h->device));
     if (h->timing) {
-        const size_t need = (size_t)(h->pending + iters) * (kGroups + 1);
-        while (h->events.size() < need) {
-            hipEvent_t e;
-            GFA2_TRY(hipEventCreate(&e));
-            h->events.push_back(e);
-        }
-        for (int i = 0; i < iters; ++i) {
-            enqueue_iteration(h, &h->events[(size_t)(h->pending) * (kGroups + 1)]);
-            h->pending++;
+        // events are recycled: at most kTimedBatch iterations are pending between reads
+        constexpr int kTimedBatch = 1024;
+        int left = iters;
+        while (left > 0) {
+            if (h->pending >= kTimedBatch) {
+                const int rc = drain_timing(h);
+                if (rc != GFA2_OK) return rc;
+            }
+            const int now = std::min(left, kTimedBatch - h->pending);
+            const size_t need = (size_t)(h->pending + now) * (kGroups + 1);
+            while (h->events.size() < need) {
+                hipEvent_t e;
+                GFA2_TRY(hipEventCreate(&e));
+                h->events.push_back(e);
+            }
+            for (int i = 0; i < now; ++i) {
+                enqueue_iteration(h, &h->events[(size_t)(h->pending) * (kGroups + 1)]);
+                h->pending++;
+            }
+            left -= now;
         }
         GFA2_TRY(hipGetLastError());
         return GFA2_OK;

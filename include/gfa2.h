@@ -92,7 +92,8 @@ int gfa2_set_params(gfa2_handle h, const gfa2_params* params);
 /* Replaces host staging + H2D copies of masses and edges, src/RPGPUForceAtlas2.cu:51-84,
  * :152-161 and sendGraphToGPU :262-268. Symmetric CSR: rowptr[n+1] (int64), col[rowptr[n]]
  * holds every undirected edge in both rows. mass may be NULL: then deg+1
- * (ForceAtlas2::mass, src/RPForceAtlas2.cpp:74-77). Resets the FA2 state. */
+ * (ForceAtlas2::mass, src/RPForceAtlas2.cpp:74-77). Resets the FA2 state. Limits: 1 <= n <=
+ * 2^26 nodes, rowptr[n] < 2^31 entries (GFA2_ERR_INVALID_GRAPH beyond). */
 int gfa2_set_graph(gfa2_handle h, int32_t n, const int64_t* rowptr, const int32_t* col, const float* mass);
 
 /* Same, from the reference's COO list (each undirected edge once, as
