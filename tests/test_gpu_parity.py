@@ -256,3 +256,44 @@ def test_autotune_restores_the_trajectory():
     assert st["iteration"] == want_stats["iteration"]
     assert st["interactions_total"] == want_stats["interactions_total"]
     eng.close()
+
+
+def _shape_case(kind):
+    rng = np.random.default_rng(11)
+    if kind == "star":                       # one hub of degree 5000: the block-per-node class
+        n = 5001
+        s, t = np.zeros(n - 1, np.int64), np.arange(1, n)
+        g = graphs.edges_to_graph(s, t)
+        pos = graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 3)
+    elif kind == "isolated":                 # nodes 300.. have no edges (mass 1, gravity only)
+        s, t = graphs.barabasi_albert(300, 3, 2)
+        g0 = graphs.edges_to_graph(s, t)
+        g = graphs.Graph(600, g0.src, g0.dst, np.arange(600))
+        pos = graphs.uniform_positions(600, 40, 4)
+    elif kind == "line":                     # every body on one vertical line
+        s, t = graphs.barabasi_albert(800, 2, 5)
+        g = graphs.edges_to_graph(s, t)
+        pos = np.stack([np.full(g.n, 3.0), rng.uniform(-50, 50, g.n)], 1).astype(np.float32)
+    else:                                    # a dense clump inside a sparse cloud
+        s, t = graphs.barabasi_albert(2000, 3, 6)
+        g = graphs.edges_to_graph(s, t)
+        pos = rng.uniform(-100, 100, (g.n, 2)).astype(np.float32)
+        pos[:500] = rng.normal(0, 1e-3, (500, 2)).astype(np.float32)
+    return g, pos
+
+
+@pytest.mark.parametrize("kind", ["star", "isolated", "line", "clump"])
+def test_forces_on_awkward_shapes(oracle, kind):
+    """Degree and position extremes: a 5000-leaf star (attraction's block-per-node class),
+    isolated nodes, collinear bodies (a zero-width bounding box side), and a clump 10^5 times
+    denser than its surroundings (deep tree levels)."""
+    print(f"case forces shape={kind}", flush=True)
+    g, pos = _shape_case(kind)
+    out = engine_first_step(g, pos)
+    T = out["tree"]
+    assert T["count"][0] == g.n and sorted(T["perm"].tolist()) == list(range(g.n))
+    rep, info = oracle.bh_repulsion(pos, g.mass(), 10.0, group=32)
+    assert rel_l2(out["rep"], rep) <= 1e-4
+    fga = oracle.attraction_coo(pos, g.src, g.dst, oracle.gravity(pos, g.mass(), False, 1.0))
+    assert rel_l2(out["fga"], fga) <= 1e-5
+    assert np.all(np.isfinite(out["pos"]))
