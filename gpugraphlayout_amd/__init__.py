@@ -433,7 +433,7 @@ class BatchEngine(Engine):
 
 MPNN_SYMBOLS = (
     "gfa2_mpnn_num_params", "gfa2_mpnn_create", "gfa2_mpnn_get_params", "gfa2_mpnn_set_params",
-    "gfa2_mpnn_add_graph", "gfa2_mpnn_features", "gfa2_mpnn_loss_grad", "gfa2_mpnn_train",
+    "gfa2_mpnn_add_graph", "gfa2_mpnn_add_engine_graph", "gfa2_mpnn_release_graph", "gfa2_mpnn_features", "gfa2_mpnn_loss_grad", "gfa2_mpnn_train",
     "gfa2_mpnn_target_scale", "gfa2_mpnn_infer", "gfa2_mpnn_infer_to_engine", "gfa2_mpnn_destroy",
 )
 ABI_SYMBOLS = ABI_SYMBOLS + MPNN_SYMBOLS
@@ -448,6 +448,8 @@ def _bind_mpnn(L):
     L.gfa2_mpnn_get_params.argtypes = [_P, _P]
     L.gfa2_mpnn_set_params.argtypes = [_P, _P]
     L.gfa2_mpnn_add_graph.argtypes = [_P, _I32, _P, _P, _P, ctypes.c_uint64, _I32, _P]
+    L.gfa2_mpnn_add_engine_graph.argtypes = [_P, _P, ctypes.c_uint64, _I32, _P]
+    L.gfa2_mpnn_release_graph.argtypes = [_P, _I32]
     L.gfa2_mpnn_features.argtypes = [_P, _I32, _P]
     L.gfa2_mpnn_loss_grad.argtypes = [_P, _I32, _P, _I32, _P, _P]
     L.gfa2_mpnn_train.argtypes = [_P, _I32, ctypes.c_float, _I32, ctypes.c_uint64, _P]
@@ -507,8 +509,26 @@ class Mpnn:
         self.sizes[gid.value] = int(n)
         return gid.value
 
+    def add_engine_graph(self, engine: "Engine", feature_seed: int = 0, steps: int = 2) -> int:
+        """Registers the graph `engine` holds (its device CSR, no host copy); no target."""
+        gid = ctypes.c_int32(0)
+        _check(lib().gfa2_mpnn_add_engine_graph(self.h, engine.h, int(feature_seed), int(steps), ctypes.byref(gid)),
+               "gfa2_mpnn_add_engine_graph")
+        self.sizes[gid.value] = engine.n
+        return gid.value
+
+    def _size(self, gid) -> int:
+        if gid not in self.sizes:
+            raise GFA2Error(-3, f"graph id {gid} is not registered")
+        return self.sizes[gid]
+
+    def release_graph(self, gid):
+        """Frees a registered graph's device memory; its id is invalid afterwards."""
+        _check(lib().gfa2_mpnn_release_graph(self.h, int(gid)), "gfa2_mpnn_release_graph")
+        self.sizes.pop(gid, None)
+
     def features(self, gid) -> np.ndarray:
-        out = np.zeros((self.sizes[gid], self.IN), np.float32)
+        out = np.zeros((self._size(gid), self.IN), np.float32)
         _check(lib().gfa2_mpnn_features(self.h, gid, _ptr(out)), "gfa2_mpnn_features")
         return out
 
@@ -530,7 +550,7 @@ class Mpnn:
         return float(lib().gfa2_mpnn_target_scale(self.h))
 
     def infer(self, gid, scale) -> np.ndarray:
-        out = np.zeros((self.sizes[gid], 2), np.float32)
+        out = np.zeros((self._size(gid), 2), np.float32)
         _check(lib().gfa2_mpnn_infer(self.h, gid, float(scale), _ptr(out)), "gfa2_mpnn_infer")
         return out
 

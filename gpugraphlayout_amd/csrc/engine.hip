@@ -1135,6 +1135,10 @@ bool engine_view(gfa2_engine* h, EngineView* v) {
     v->device = h->device;
     v->pos = h->b.pos;
     v->stream = h->stream;
+    v->rowptr = h->b.rowptr;
+    v->col = h->b.col;
+    v->nnz = h->b.nnz;
+    v->nseg = h->b.nseg;
     return true;
 }
 

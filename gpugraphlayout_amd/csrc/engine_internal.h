@@ -213,6 +213,10 @@ struct EngineView {
     int device;
     float2* pos;
     hipStream_t stream;
+    const int* rowptr;      // the graph's device CSR
+    const int* col;
+    long long nnz;
+    int nseg;
 };
 bool engine_view(struct ::gfa2_engine* h, EngineView* v);
 // The buffers, device and stream of an engine holding a graph and positions (rasteriser).

@@ -61,18 +61,6 @@ __global__ __launch_bounds__(256) void k_normal_fields(int n, unsigned long long
 }
 
 // out[i][f] = (in[i][f] + mean_j in[j][f]) / 2 for 8 columns.
-__global__ __launch_bounds__(256) void k_smooth8(const int* __restrict__ rowptr, const int* __restrict__ col, int n,
-                                                 const float* __restrict__ in, float* __restrict__ out) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n * 8) return;
-    const int i = t >> 3, f = t & 7;
-    const int e0 = rowptr[i], e1 = rowptr[i + 1];
-    float s = 0.0f;
-    for (int e = e0; e < e1; ++e) s += in[col[e] * 8 + f];
-    const float m = e1 > e0 ? s / (float)(e1 - e0) : 0.0f;
-    out[t] = 0.5f * (in[t] + m);
-}
-
 // X (unnormalised) columns 0..8, zero padding 9..15.
 __global__ __launch_bounds__(256) void k_assemble(const int* __restrict__ rowptr, int n, const float* __restrict__ Z,
                                                   float* __restrict__ X) {
@@ -130,55 +118,174 @@ __global__ __launch_bounds__(256) void k_colnorm_apply(float* __restrict__ X, in
     X[t] = (X[t] - mu_sd[2 * f]) * mu_sd[2 * f + 1];
 }
 
-// ---- propagation ---------------------------------------------------------------------------
+// ---- neighbour aggregation -----------------------------------------------------------------
 
-// out[i][:] = scale_i * sum_{j in N(i)} w_j * in[j][:], F lanes per node (F = 16 or 64).
-// Forward mean: scale_i = 1/deg(i), w_j = 1. Transpose of the mean (backward): scale_i = 1,
-// w_j = 1/deg(j) (the CSR is symmetric).
-template <int F, bool TRANSPOSE>
-__global__ __launch_bounds__(256) void k_agg(const int* __restrict__ rowptr, const int* __restrict__ col, int n,
-                                             const float* __restrict__ in, float* __restrict__ out) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int i = t / F, f = t % F;
-    if (i >= n) return;
-    const int e0 = rowptr[i], e1 = rowptr[i + 1];
-    float s = 0.0f;
-    for (int e = e0; e < e1; ++e) {
-        const int j = col[e];
-        const float v = in[j * F + f];
-        if (TRANSPOSE) s = fmaf(v, 1.0f / (float)(rowptr[j + 1] - rowptr[j]), s);
-        else s += v;
-    }
-    out[t] = TRANSPOSE ? s : (e1 > e0 ? s / (float)(e1 - e0) : 0.0f);
+// out[i][:] = epilogue(sum over CSR neighbours j of in[j][:], times 1/deg(j) for the transposed
+// form), F features per node (8, 16 or 64):
+//   kAggMean       out = sum / deg(i) (0 for an isolated node)     forward mean
+//   kAggSmooth     out = (in[i] + sum / deg(i)) / 2                 feature smoothing
+//   kAggTranspose  out = sum of in[j] / deg(j)                      backward of the mean
+// Rows of at most kHeavyDeg entries: F lanes per row, entries in CSR order (four loads in
+// flight). Longer rows (the hubs of a skewed graph, which would otherwise leave one wave
+// looping over 10^5 entries) are split into chunks of kChunk entries: one wave per chunk sums
+// its entries in 64/F interleaved slots, combined in slot order, into a partial; then one
+// thread per (row, feature) adds the row's partials in chunk order. Every sum has a fixed
+// order, so results are deterministic.
+enum AggMode { kAggMean = 0, kAggSmooth = 1, kAggTranspose = 2 };
+
+template <int MODE>
+__device__ __forceinline__ float agg_add(float s, const int* __restrict__ rowptr, const float* __restrict__ in,
+                                         int j, int F, int f) {
+    const float v = in[(size_t)j * F + f];
+    if (MODE == kAggTranspose) return fmaf(v, 1.0f / (float)(rowptr[j + 1] - rowptr[j]), s);
+    return s + v;
 }
 
-// H[i][o] = act(sum_k A[i][k] Ws[k][o] + sum_k B[i][k] Wn[k][o] + b[o]); one wave per node,
-// lane o; weights (BF16) staged in LDS as FP32; the node's input row broadcast by shuffle.
+template <int MODE>
+__device__ __forceinline__ float agg_finish(float s, int deg, float self) {
+    if (MODE == kAggTranspose) return s;
+    const float m = deg > 0 ? s / (float)deg : 0.0f;
+    return MODE == kAggSmooth ? 0.5f * (self + m) : m;
+}
+
+template <int F, int MODE>
+__global__ __launch_bounds__(256) void k_agg_rows(const int* __restrict__ rowptr, const int* __restrict__ col, int n,
+                                                  const float* __restrict__ in, float* __restrict__ out) {
+    constexpr int kRows = 256 / F;
+    const int f = threadIdx.x % F;
+    for (long long r0 = (long long)blockIdx.x * kRows; r0 < n; r0 += (long long)gridDim.x * kRows) {
+        const long long ii = r0 + threadIdx.x / F;
+        if (ii >= n) continue;
+        const int i = (int)ii;
+        const int e0 = rowptr[i], e1 = rowptr[i + 1];
+        if (e1 - e0 > kHeavyDeg) continue;            // the chunk path writes this row
+        float s = 0.0f;
+        int e = e0;
+        for (; e + 4 <= e1; e += 4) {
+            const int j0 = col[e], j1 = col[e + 1], j2 = col[e + 2], j3 = col[e + 3];
+            s = agg_add<MODE>(s, rowptr, in, j0, F, f);
+            s = agg_add<MODE>(s, rowptr, in, j1, F, f);
+            s = agg_add<MODE>(s, rowptr, in, j2, F, f);
+            s = agg_add<MODE>(s, rowptr, in, j3, F, f);
+        }
+        for (; e < e1; ++e) s = agg_add<MODE>(s, rowptr, in, col[e], F, f);
+        out[(size_t)i * F + f] = agg_finish<MODE>(s, e1 - e0, MODE == kAggSmooth ? in[(size_t)i * F + f] : 0.0f);
+    }
+}
+
+// One wave per chunk {row, first entry}; part[c][f] = the chunk's sum.
+template <int F, int MODE>
+__global__ __launch_bounds__(256) void k_agg_chunks(const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                    const int2* __restrict__ chunks, int nchunks,
+                                                    const float* __restrict__ in, float* __restrict__ part) {
+    constexpr int S = 64 / F;
+    const int lane = threadIdx.x & 63, slot = lane / F, f = lane % F;
+    for (int c = blockIdx.x * 4 + (threadIdx.x >> 6); c < nchunks; c += gridDim.x * 4) {
+        const int2 ch = chunks[c];
+        const int end = min(ch.y + kChunk, rowptr[ch.x + 1]);
+        float s = 0.0f;
+        int e = ch.y + slot;
+        for (; e + 3 * S < end; e += 4 * S) {
+            const int j0 = col[e], j1 = col[e + S], j2 = col[e + 2 * S], j3 = col[e + 3 * S];
+            s = agg_add<MODE>(s, rowptr, in, j0, F, f);
+            s = agg_add<MODE>(s, rowptr, in, j1, F, f);
+            s = agg_add<MODE>(s, rowptr, in, j2, F, f);
+            s = agg_add<MODE>(s, rowptr, in, j3, F, f);
+        }
+        for (; e < end; e += S) s = agg_add<MODE>(s, rowptr, in, col[e], F, f);
+        float tot = 0.0f;
+#pragma unroll
+        for (int q = 0; q < S; ++q) tot += __shfl(s, q * F + f, 64);
+        if (slot == 0) part[(size_t)c * F + f] = tot;
+    }
+}
+
+// One thread per (heavy row, feature): the row's chunk partials in chunk order.
+template <int F, int MODE>
+__global__ __launch_bounds__(256) void k_agg_heavy(const int* __restrict__ rowptr, const int* __restrict__ heavy_row,
+                                                   const int* __restrict__ heavy_c0, int nheavy,
+                                                   const float* __restrict__ in, const float* __restrict__ part,
+                                                   float* __restrict__ out) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nheavy * F) return;
+    const int h = (int)(t / F), f = (int)(t % F);
+    const int i = heavy_row[h];
+    float s = 0.0f;
+    for (int c = heavy_c0[h]; c < heavy_c0[h + 1]; ++c) s += part[(size_t)c * F + f];
+    out[(size_t)i * F + f] = agg_finish<MODE>(s, rowptr[i + 1] - rowptr[i],
+                                              MODE == kAggSmooth ? in[(size_t)i * F + f] : 0.0f);
+}
+
+// H[i][o] = relu(b[o] + sum_k (A[i][k] Ws[k][o] + B[i][k] Wn[k][o])), accumulated in the
+// order b, A_0 Ws_0, B_0 Wn_0, A_1 Ws_1, ... (FMAs). Register-tiled: a block holds the BF16
+// weights as FP32 in LDS and walks tiles of 64 nodes, staging A and B transposed in LDS; each
+// thread owns 4 nodes x 4 outputs (16 accumulators), so one k step is 4 LDS reads of 16 bytes
+// for 32 FMAs.
 template <int K>
 __global__ __launch_bounds__(256) void k_dense(const float* __restrict__ A, const float* __restrict__ B, int n,
                                                const unsigned short* __restrict__ Ws,
                                                const unsigned short* __restrict__ Wn,
-                                               const unsigned short* __restrict__ bias, float* __restrict__ Z,
-                                               float* __restrict__ H) {
-    __shared__ float ws[K * kHid], wn[K * kHid];
-    for (int k = threadIdx.x; k < K * kHid; k += 256) {
-        ws[k] = bf16_to_f32(Ws[k]);
-        wn[k] = bf16_to_f32(Wn[k]);
+                                               const unsigned short* __restrict__ bias, float* __restrict__ H) {
+    __shared__ float4 ws[K * kHid / 4], wn[K * kHid / 4];     // [k][o/4]
+    __shared__ float4 ta[K * 16], tb[K * 16];                 // [k][node/4], 64 nodes
+    for (int q = threadIdx.x; q < K * kHid; q += 256) {
+        reinterpret_cast<float*>(ws)[q] = bf16_to_f32(Ws[q]);
+        reinterpret_cast<float*>(wn)[q] = bf16_to_f32(Wn[q]);
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const float bo = bf16_to_f32(bias[lane]);
-    for (int i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += gridDim.x * 4) {
-        const float a = lane < K ? A[i * K + lane] : 0.0f;
-        const float b = lane < K ? B[i * K + lane] : 0.0f;
-        float z = bo;
-#pragma unroll 16
-        for (int k = 0; k < K; ++k) {
-            z = fmaf(__shfl(a, k, 64), ws[k * kHid + lane], z);
-            z = fmaf(__shfl(b, k, 64), wn[k * kHid + lane], z);
+    const int oq = threadIdx.x & 15, rq = threadIdx.x >> 4;   // outputs 4*oq.., nodes 4*rq..
+    const float4 bo = make_float4(bf16_to_f32(bias[4 * oq]), bf16_to_f32(bias[4 * oq + 1]),
+                                  bf16_to_f32(bias[4 * oq + 2]), bf16_to_f32(bias[4 * oq + 3]));
+    const int tiles = (n + 63) / 64;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const long long i0 = (long long)tile * 64;
+        __syncthreads();                                      // weights written / last tile read
+        for (int q = threadIdx.x; q < 64 * (K / 4); q += 256) {
+            const int r = q & 63, k4 = q >> 6;                // node r, features 4*k4..
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+            if (i0 + r < n) {
+                a = reinterpret_cast<const float4*>(A + (i0 + r) * K)[k4];
+                b = reinterpret_cast<const float4*>(B + (i0 + r) * K)[k4];
+            }
+            float* fa = reinterpret_cast<float*>(ta);
+            float* fb = reinterpret_cast<float*>(tb);
+            fa[(4 * k4 + 0) * 64 + r] = a.x;
+            fa[(4 * k4 + 1) * 64 + r] = a.y;
+            fa[(4 * k4 + 2) * 64 + r] = a.z;
+            fa[(4 * k4 + 3) * 64 + r] = a.w;
+            fb[(4 * k4 + 0) * 64 + r] = b.x;
+            fb[(4 * k4 + 1) * 64 + r] = b.y;
+            fb[(4 * k4 + 2) * 64 + r] = b.z;
+            fb[(4 * k4 + 3) * 64 + r] = b.w;
         }
-        if (Z) Z[i * kHid + lane] = z;
-        H[i * kHid + lane] = fmaxf(z, 0.0f);
+        __syncthreads();
+        float z[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            z[r][0] = bo.x;
+            z[r][1] = bo.y;
+            z[r][2] = bo.z;
+            z[r][3] = bo.w;
+        }
+#pragma unroll 4
+        for (int k = 0; k < K; ++k) {
+            const float4 a = ta[k * 16 + rq], b = tb[k * 16 + rq];
+            const float4 u = ws[k * 16 + oq], v = wn[k * 16 + oq];
+            const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                z[r][0] = fmaf(bv[r], v.x, fmaf(av[r], u.x, z[r][0]));
+                z[r][1] = fmaf(bv[r], v.y, fmaf(av[r], u.y, z[r][1]));
+                z[r][2] = fmaf(bv[r], v.z, fmaf(av[r], u.z, z[r][2]));
+                z[r][3] = fmaf(bv[r], v.w, fmaf(av[r], u.w, z[r][3]));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long long i = i0 + 4 * rq + r;
+            if (i < n)
+                reinterpret_cast<float4*>(H + i * kHid)[oq] =
+                    make_float4(fmaxf(z[r][0], 0.f), fmaxf(z[r][1], 0.f), fmaxf(z[r][2], 0.f), fmaxf(z[r][3], 0.f));
+        }
     }
 }
 
@@ -411,7 +518,33 @@ int blocks_for(long long work) {
     return (int)(b < 1 ? 1 : b);
 }
 
+template <int F, int MODE>
+void aggregate(const MpnnGraph& g, const float* in, float* out, hipStream_t st) {
+    const long long rb = ((long long)g.n * F + 255) / 256;
+    k_agg_rows<F, MODE><<<(int)(rb < 1 ? 1 : (rb > 65536 ? 65536 : rb)), 256, 0, st>>>(g.rowptr, g.col, g.n, in, out);
+    if (g.nchunks > 0) {
+        const int cb = (g.nchunks + 3) / 4;
+        k_agg_chunks<F, MODE><<<cb < 8192 ? cb : 8192, 256, 0, st>>>(g.rowptr, g.col, g.chunks, g.nchunks, in, g.part);
+        k_agg_heavy<F, MODE><<<blocks_for((long long)g.nheavy * F), 256, 0, st>>>(g.rowptr, g.heavy_row, g.heavy_c0,
+                                                                                 g.nheavy, in, g.part, out);
+    }
+}
+
 }  // namespace
+
+void mpnn_agg_plan(const int* rowptr, int n, std::vector<int>& heavy_row, std::vector<int>& heavy_c0,
+                   std::vector<int2>& chunks) {
+    heavy_row.clear();
+    heavy_c0.assign(1, 0);
+    chunks.clear();
+    for (int i = 0; i < n; ++i) {
+        const int e0 = rowptr[i], e1 = rowptr[i + 1];
+        if (e1 - e0 <= kHeavyDeg) continue;
+        heavy_row.push_back(i);
+        for (int e = e0; e < e1; e += kChunk) chunks.push_back(make_int2(i, e));
+        heavy_c0.push_back((int)chunks.size());
+    }
+}
 
 void mpnn_features(const MpnnGraph& g, unsigned long long seed, int steps, MpnnScratch& s, hipStream_t st) {
     const int n = g.n;
@@ -419,7 +552,7 @@ void mpnn_features(const MpnnGraph& g, unsigned long long seed, int steps, MpnnS
     float* a = s.z0;
     float* b = s.z1;
     for (int k = 0; k < steps; ++k) {
-        k_smooth8<<<blocks_for((long long)n * 8), 256, 0, st>>>(g.rowptr, g.col, n, a, b);
+        aggregate<8, kAggSmooth>(g, a, b, st);
         float* t = a;
         a = b;
         b = t;
@@ -429,15 +562,15 @@ void mpnn_features(const MpnnGraph& g, unsigned long long seed, int steps, MpnnS
     k_colstats<<<nb, 256, 0, st>>>(g.X, n, s.dpart);
     k_colnorm_finish<<<1, 64, 0, st>>>(s.dpart, nb, n, s.musd);
     k_colnorm_apply<<<blocks_for((long long)n * kIn), 256, 0, st>>>(g.X, n, s.musd);
-    k_agg<kIn, false><<<blocks_for((long long)n * kIn), 256, 0, st>>>(g.rowptr, g.col, n, g.X, g.A0);
+    aggregate<kIn, kAggMean>(g, g.X, g.A0, st);
 }
 
 void mpnn_forward(const MpnnGraph& g, const MpnnWeights& w, MpnnScratch& s, hipStream_t st) {
     const int n = g.n;
-    const int db = (n + 3) / 4 < 2048 ? (n + 3) / 4 : 2048;
-    k_dense<kIn><<<db, 256, 0, st>>>(g.X, g.A0, n, w.bf + kOffW1s, w.bf + kOffW1n, w.bf + kOffB1, nullptr, s.H1);
-    k_agg<kHid, false><<<blocks_for((long long)n * kHid), 256, 0, st>>>(g.rowptr, g.col, n, s.H1, s.A1);
-    k_dense<kHid><<<db, 256, 0, st>>>(s.H1, s.A1, n, w.bf + kOffW2s, w.bf + kOffW2n, w.bf + kOffB2, nullptr, s.H2);
+    const int db = (n + 63) / 64 < 1024 ? (n + 63) / 64 : 1024;
+    k_dense<kIn><<<db, 256, 0, st>>>(g.X, g.A0, n, w.bf + kOffW1s, w.bf + kOffW1n, w.bf + kOffB1, s.H1);
+    aggregate<kHid, kAggMean>(g, s.H1, s.A1, st);
+    k_dense<kHid><<<db, 256, 0, st>>>(s.H1, s.A1, n, w.bf + kOffW2s, w.bf + kOffW2n, w.bf + kOffB2, s.H2);
     k_head<<<blocks_for(n), 256, 0, st>>>(s.H2, n, w.bf + kOffWo, w.bf + kOffBo, s.Y);
 }
 
@@ -466,7 +599,7 @@ int mpnn_loss_backward(const MpnnGraph& g, const MpnnWeights& w, MpnnScratch& s,
     const int db = (n + 3) / 4 < 2048 ? (n + 3) / 4 : 2048;
     k_dense_back_x<<<db, 256, 0, st>>>(s.dZ2, n, w.bf + kOffW2s, s.P);
     k_dense_back_x<<<db, 256, 0, st>>>(s.dZ2, n, w.bf + kOffW2n, s.Q);
-    k_agg<kHid, true><<<blocks_for((long long)n * kHid), 256, 0, st>>>(g.rowptr, g.col, n, s.Q, s.A1);
+    aggregate<kHid, kAggTranspose>(g, s.Q, s.A1, st);
     k_add_mask<<<blocks_for((long long)n * kHid), 256, 0, st>>>(s.P, s.A1, s.H1, n * kHid, s.dZ1);
     // layer 1
     k_wgrad<kIn><<<chunks, 256, 0, st>>>(g.X, s.dZ1, n, s.wpart);

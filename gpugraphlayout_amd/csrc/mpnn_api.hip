@@ -78,8 +78,29 @@ int ensure_scratch(gfa2_mpnn* m, int n) {
     return GFA2_OK;
 }
 
+// Heavy rows and chunks of a graph's aggregation, from its row offsets on the host.
+int upload_agg_plan(gfa2_mpnn* m, const int* rp, MpnnGraph& g, std::vector<void*>& mine) {
+    std::vector<int> heavy_row, heavy_c0;
+    std::vector<int2> chunks;
+    mpnn_agg_plan(rp, g.n, heavy_row, heavy_c0, chunks);
+    g.nheavy = (int)heavy_row.size();
+    g.nchunks = (int)chunks.size();
+    if (g.nchunks == 0) return GFA2_OK;
+    int st;
+    if ((st = mp_alloc(mine, &g.heavy_row, heavy_row.size())) != GFA2_OK ||
+        (st = mp_alloc(mine, &g.heavy_c0, heavy_c0.size())) != GFA2_OK ||
+        (st = mp_alloc(mine, &g.chunks, chunks.size())) != GFA2_OK ||
+        (st = mp_alloc(mine, &g.part, chunks.size() * (size_t)kHid)) != GFA2_OK)
+        return st;
+    MP_TRY(hipMemcpyAsync(g.heavy_row, heavy_row.data(), sizeof(int) * heavy_row.size(), hipMemcpyHostToDevice, m->stream));
+    MP_TRY(hipMemcpyAsync(g.heavy_c0, heavy_c0.data(), sizeof(int) * heavy_c0.size(), hipMemcpyHostToDevice, m->stream));
+    MP_TRY(hipMemcpyAsync(g.chunks, chunks.data(), sizeof(int2) * chunks.size(), hipMemcpyHostToDevice, m->stream));
+    MP_TRY(hipStreamSynchronize(m->stream));   // the host vectors go out of scope
+    return GFA2_OK;
+}
+
 int check_graph(const gfa2_mpnn* m, int id) {
-    if (!m || id < 0 || id >= (int)m->graphs.size()) return GFA2_ERR_INVALID_ARG;
+    if (!m || id < 0 || id >= (int)m->graphs.size() || m->graphs[id].n == 0) return GFA2_ERR_INVALID_ARG;
     return GFA2_OK;
 }
 
@@ -180,6 +201,10 @@ int gfa2_mpnn_add_graph(gfa2_mpnn_t m, int32_t n, const int64_t* rowptr, const i
     }
     MP_TRY(hipMemcpyAsync(g.rowptr, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, m->stream));
     if (nnz) MP_TRY(hipMemcpyAsync(g.col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, m->stream));
+    if ((st = upload_agg_plan(m, rp.data(), g, mine)) != GFA2_OK) {
+        for (void* p : mine) (void)hipFree(p);
+        return st;
+    }
     if (target_xy) {
         // centre and scale to unit RMS radius
         double cx = 0.0, cy = 0.0;
@@ -213,6 +238,58 @@ int gfa2_mpnn_add_graph(gfa2_mpnn_t m, int32_t n, const int64_t* rowptr, const i
     m->graphs.push_back(g);
     m->graph_allocs.push_back(mine);
     if (graph_id) *graph_id = (int32_t)m->graphs.size() - 1;
+    return GFA2_OK;
+}
+
+int gfa2_mpnn_add_engine_graph(gfa2_mpnn_t m, gfa2_handle h, uint64_t feature_seed, int32_t smoothing_steps,
+                               int32_t* graph_id) {
+    if (!m || !h || smoothing_steps < 0) return GFA2_ERR_INVALID_ARG;
+    EngineView v;
+    if (!engine_view(h, &v)) return GFA2_ERR_NO_GRAPH;
+    if (v.nseg > 1) return GFA2_ERR_UNSUPPORTED;
+    if (v.device != m->device) return GFA2_ERR_INVALID_ARG;
+    MP_TRY(hipSetDevice(m->device));
+    const int n = v.n;
+    int st = ensure_scratch(m, n);
+    if (st != GFA2_OK) return st;
+    MpnnGraph g;
+    g.n = n;
+    g.nnz = v.nnz;
+    std::vector<void*> mine;
+    if ((st = mp_alloc(mine, &g.rowptr, (size_t)n + 1)) != GFA2_OK || (st = mp_alloc(mine, &g.col, (size_t)g.nnz)) != GFA2_OK ||
+        (st = mp_alloc(mine, &g.X, (size_t)n * kIn)) != GFA2_OK || (st = mp_alloc(mine, &g.A0, (size_t)n * kIn)) != GFA2_OK) {
+        for (void* p : mine) (void)hipFree(p);
+        return st;
+    }
+    // the engine's CSR was validated when it was set; copy it device to device after the
+    // engine's queued work
+    MP_TRY(hipStreamSynchronize(v.stream));
+    MP_TRY(hipMemcpyAsync(g.rowptr, v.rowptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, m->stream));
+    if (g.nnz) MP_TRY(hipMemcpyAsync(g.col, v.col, sizeof(int) * (size_t)g.nnz, hipMemcpyDeviceToDevice, m->stream));
+    std::vector<int> rp((size_t)n + 1);
+    MP_TRY(hipMemcpyAsync(rp.data(), v.rowptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, m->stream));
+    MP_TRY(hipStreamSynchronize(m->stream));
+    if ((st = upload_agg_plan(m, rp.data(), g, mine)) != GFA2_OK) {
+        for (void* p : mine) (void)hipFree(p);
+        return st;
+    }
+    mpnn_features(g, feature_seed, smoothing_steps, m->s, m->stream);
+    MP_TRY(hipGetLastError());
+    MP_TRY(hipStreamSynchronize(m->stream));
+    m->graphs.push_back(g);
+    m->graph_allocs.push_back(mine);
+    if (graph_id) *graph_id = (int32_t)m->graphs.size() - 1;
+    return GFA2_OK;
+}
+
+int gfa2_mpnn_release_graph(gfa2_mpnn_t m, int32_t graph_id) {
+    int st = check_graph(m, graph_id);
+    if (st != GFA2_OK) return st;
+    MP_TRY(hipSetDevice(m->device));
+    MP_TRY(hipStreamSynchronize(m->stream));
+    for (void* p : m->graph_allocs[graph_id]) (void)hipFree(p);
+    m->graph_allocs[graph_id].clear();
+    m->graphs[graph_id] = MpnnGraph{};     // n = 0, no target: skipped by training, rejected by id
     return GFA2_OK;
 }
 

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace gfa2 {
 
 constexpr int kIn = 16;      // input features (9 used, zero padded)
@@ -17,11 +19,21 @@ constexpr int kOffWo = kOffB2 + kHid;
 constexpr int kOffBo = kOffWo + kHid * 2;
 constexpr int kParams = kOffBo + 2;
 
+// Rows longer than kHeavyDeg entries are aggregated in chunks of kChunk entries.
+constexpr int kHeavyDeg = 512;
+constexpr int kChunk = 512;
+
 struct MpnnGraph {
     int n = 0;
     long long nnz = 0;
     int* rowptr = nullptr;
     int* col = nullptr;
+    int nheavy = 0;             // rows with more than kHeavyDeg entries
+    int nchunks = 0;
+    int* heavy_row = nullptr;   // [nheavy]
+    int* heavy_c0 = nullptr;    // [nheavy + 1] first chunk of each heavy row
+    int2* chunks = nullptr;     // [nchunks] {row, first entry}
+    float* part = nullptr;      // [nchunks x kHid] chunk partial sums
     float* X = nullptr;     // [n x kIn] features
     float* A0 = nullptr;    // [n x kIn] mean_agg(X)
     float* T = nullptr;     // [n x 2] normalised target, or null
@@ -47,6 +59,9 @@ struct MpnnScratch {
     float* gtmp = nullptr;      // (kHid+1) * kHid
 };
 
+// Host side: the heavy rows and chunk list of a CSR given by its row offsets.
+void mpnn_agg_plan(const int* rowptr, int n, std::vector<int>& heavy_row, std::vector<int>& heavy_c0,
+                   std::vector<int2>& chunks);
 void mpnn_features(const MpnnGraph& g, unsigned long long seed, int steps, MpnnScratch& s, hipStream_t st);
 void mpnn_forward(const MpnnGraph& g, const MpnnWeights& w, MpnnScratch& s, hipStream_t st);
 int mpnn_loss_backward(const MpnnGraph& g, const MpnnWeights& w, MpnnScratch& s, const int* d_offs, int R,

@@ -1,19 +1,38 @@
 """Learned initial layouts for ForceAtlas2: training data, training, and the
 time-to-equal-stress comparison against plain (random-start) FA2.
 
-Workload: stochastic block model graphs (50,000 nodes, 32 blocks with Dirichlet(2) sizes,
-p_in = 8/|B|, p_out = 0.5/V), training seeds 1000.., held-out seeds 5000..; targets are
-plain FA2 layouts after 1,000 iterations (weak gravity k_g = 1, k_r = 2, theta = 1),
-computed as batched layouts. The network (gfa2_mpnn_*) is trained on the device.
+Two workloads:
 
-Evaluation per held-out graph:
-  plain   : random start (uniform square of side 2*sqrt(V)), N FA2 iterations; wall time
-            of the iterations; final normalized stress NS_plain on 10^6 seeded pairs.
-  learned : features + forward pass on the device written into the engine, then FA2
-            until NS <= NS_plain. The iteration count k* is found with stress checks every
-            `check` iterations; the wall time is then measured on a clean rerun of
-            inference + k* iterations (no checks inside the timed region).
-Usage: python -m gpugraphlayout_amd.learned --train 200 --test 20 --out result.json
+  sbm50k  stochastic block model graphs (50,000 nodes, 32 blocks with Dirichlet(2) sizes,
+          p_in = 8/|B|, p_out = 0.5/V); training seeds 1000.., held-out seeds 5000..;
+          FA2 with weak gravity k_g = 1, k_r = 2, theta = 1; plain run 1,000 iterations.
+  rmat    the scale test: trained on small R-MAT graphs (2^14..2^17 ids, 8 edges per id,
+          (0.57, 0.19, 0.19, 0.05), seeds 1000..), tested on R-MAT 2^24 with 128M edges
+          (seed 3) and on R-MAT 2^20 with 10M edges (seed 1); FA2 with weak gravity
+          k_g = 1, k_r = 10; plain run 200 (2^24) or 500 (2^20) iterations from the
+          reference's glibc start (srandom(1234), square of side 2*sqrt(V)).
+
+Targets are plain FA2 layouts after 1,000 iterations from a uniform random start, computed
+as batched layouts; the network (gfa2_mpnn_*) is trained on the device.
+
+Evaluation per test graph:
+  plain   : random start, N FA2 iterations (hipGraph replay); wall time of the iterations;
+            final normalized stress NS_plain on 10^6 seeded pairs.
+  learned : features (from the engine's device CSR) + forward pass written into the
+            engine, then FA2 until NS <= NS_plain. The iteration count k* is found with
+            stress checks every `check` iterations; the wall time is then measured on a
+            clean rerun of features + inference + k* iterations (no checks inside the
+            timed region).
+
+The learned start is placed at RMS radius kappa * c * sqrt(k_r * E): c is the mean of
+rms / sqrt(k_r * E) over the training targets (FA2's equilibrium size grows like the square
+root of k_r times the edge count), kappa a start compression picked on validation graphs
+(`--validate`: R-MAT 2^24 with 128M edges, seed 4, 200 iterations), never on the test
+graphs.
+
+Usage: python -m gpugraphlayout_amd.learned --workload sbm50k --train 200 --test 20 --out r.json
+       python -m gpugraphlayout_amd.learned --workload rmat --out r.json
+       python -m gpugraphlayout_amd.learned --workload rmat --validate
 """
 import argparse
 import json
@@ -24,7 +43,12 @@ import numpy as np
 
 from . import BatchEngine, Engine, Mpnn, graphs
 
-FA2 = dict(scale=2.0, gravity=1.0, strong_gravity=0)
+FA2 = dict(scale=2.0, gravity=1.0, strong_gravity=0)          # sbm50k
+FA2_RMAT = dict(scale=10.0, gravity=1.0, strong_gravity=0)    # rmat
+
+# defaults picked by validation (feature smoothing rounds, start compression kappa)
+DEFAULTS = {"sbm50k": {"feature_steps": 2, "kappa": 0.5, "train": 200, "check": 10, "test": 100},
+            "rmat": {"feature_steps": 1, "kappa": 0.5, "train": 96, "check": 5, "test": 0}}
 
 
 def sbm_graph(seed, n=50_000, blocks=32, alpha=2.0):
@@ -32,12 +56,26 @@ def sbm_graph(seed, n=50_000, blocks=32, alpha=2.0):
     return graphs.edges_to_graph(s, t)
 
 
+def rmat_graph(scale, edges, seed):
+    """R-MAT graph with UGraph semantics through the C++ host loader (ids in first-seen order)."""
+    from .host import HostGraph
+    s, t = graphs.rmat(scale, edges, seed=seed)
+    hg = HostGraph.from_edges(s, t)
+    return hg.arrays()
+
+
+def rmat_training_graph(k, seed0=1000):
+    sc = 14 + (k % 4)
+    return rmat_graph(sc, 8 << sc, seed0 + k)
+
+
 def start_positions(g, seed):
     return graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), seed)
 
 
-def fa2_targets(gs, iters, chunk_nodes=4_000_000, seed0=0, log=None):
+def fa2_targets(gs, iters, params=None, chunk_nodes=4_000_000, seed0=0, log=None):
     """Plain FA2 layouts after `iters` iterations, many graphs per batched launch chain."""
+    params = FA2 if params is None else params
     out = [None] * len(gs)
     k = 0
     while k < len(gs):
@@ -45,7 +83,7 @@ def fa2_targets(gs, iters, chunk_nodes=4_000_000, seed0=0, log=None):
         while j < len(gs) and (j == k or total + gs[j].n <= chunk_nodes):
             total += gs[j].n
             j += 1
-        be = BatchEngine([(g.n, g.src, g.dst) for g in gs[k:j]], **FA2)
+        be = BatchEngine([(g.n, g.src, g.dst) for g in gs[k:j]], **params)
         be.set_graph_capture(True)
         be.set_positions(np.concatenate([start_positions(g, seed0 + k + q) for q, g in enumerate(gs[k:j])]))
         be.step(iters)
@@ -57,6 +95,12 @@ def fa2_targets(gs, iters, chunk_nodes=4_000_000, seed0=0, log=None):
             log(f"targets {j}/{len(gs)}")
         k = j
     return out
+
+
+def size_constant(gs, targets, k_r):
+    """Mean over the training targets of rms radius / sqrt(k_r * E)."""
+    c = [np.sqrt(((p - p.mean(0)) ** 2).sum(1).mean()) / np.sqrt(k_r * g.m) for g, p in zip(gs, targets)]
+    return float(np.mean(c))
 
 
 def train(gs, targets, steps, lr=3e-3, R=8, seed=0, feature_steps=2, log=None):
@@ -74,90 +118,190 @@ def train(gs, targets, steps, lr=3e-3, R=8, seed=0, feature_steps=2, log=None):
     return m
 
 
-def evaluate(m, g, seed, iters_plain=1000, check=10, feature_steps=2, nsources=1024, ntargets=977):
-    rowptr, col = g.csr()
-    # plain run
-    e = Engine(g.n, g.src, g.dst, **FA2)
-    e.set_graph_capture(True)
-    e.stress_setup(nsources, ntargets, seed=seed)
-    e.set_positions(start_positions(g, 10_000 + seed))
+def plain_run(e, start, iters_plain):
+    """Plain FA2 from `start`: wall time of the iterations and the final normalized stress."""
+    e.set_positions(start)
+    e.reset()
     e.sync()
     t0 = time.perf_counter()
     e.step(iters_plain)
     e.sync()
     t_plain = time.perf_counter() - t0
-    ns_plain, _ = e.normalized_stress()
-    ns_random_start, _ = e.normalized_stress(start_positions(g, 10_000 + seed))
-    # learned: find k*
-    gid = m.add_graph(g.n, rowptr, col, feature_seed=10_000 + seed, steps=feature_steps)
-    scale = m.target_scale()
+    return {"plain_iterations": iters_plain, "plain_seconds": t_plain, "ns_plain": e.normalized_stress()[0],
+            "ns_random_start": e.normalized_stress(start)[0]}
+
+
+def learned_run(m, e, plain, start_scale, feature_seed, feature_steps, check):
+    """Learned start + FA2 on the same engine until NS <= the plain run's (see module doc)."""
+    ns_plain, iters_plain = plain["ns_plain"], plain["plain_iterations"]
     e.reset()
-    m.infer_to_engine(gid, scale, e)
-    ns_start, _ = e.normalized_stress()
+    gid = m.add_engine_graph(e, feature_seed=feature_seed, steps=feature_steps)
+    m.infer_to_engine(gid, start_scale, e)
+    m.release_graph(gid)
+    trajectory = [e.normalized_stress()[0]]
     kstar = None
     it = 0
-    while it <= iters_plain:
-        if e.normalized_stress()[0] <= ns_plain:
+    while True:
+        if trajectory[-1] <= ns_plain:
             kstar = it
+            break
+        if it + check > iters_plain:
             break
         e.step(check)
         it += check
-    result = {"seed": seed, "nodes": g.n, "edges": g.m, "plain_iterations": iters_plain, "plain_seconds": t_plain,
-              "ns_plain": ns_plain, "ns_random_start": ns_random_start, "ns_learned_start": ns_start,
-              "learned_iterations": kstar}
+        trajectory.append(e.normalized_stress()[0])
+    r = dict(plain)
+    r.update({"ns_learned_start": trajectory[0], "start_scale": start_scale, "feature_steps": feature_steps,
+              "learned_iterations": kstar, "learned_trajectory": trajectory, "check": check})
     if kstar is not None:
-        # clean timed rerun: features + inference + k* iterations
         e.reset()
         e.sync()
         t0 = time.perf_counter()
-        gid2 = m.add_graph(g.n, rowptr, col, feature_seed=10_000 + seed, steps=feature_steps)
-        m.infer_to_engine(gid2, scale, e)
+        gid2 = m.add_engine_graph(e, feature_seed=feature_seed, steps=feature_steps)
+        m.infer_to_engine(gid2, start_scale, e)
         e.step(kstar)
         e.sync()
         t_learned = time.perf_counter() - t0
-        result["learned_seconds"] = t_learned
-        result["ns_learned"] = e.normalized_stress()[0]
-        result["time_ratio"] = t_learned / t_plain
-    e.close()
-    return result
+        m.release_graph(gid2)
+        r["learned_seconds"] = t_learned
+        r["ns_learned"] = e.normalized_stress()[0]
+        r["time_ratio"] = t_learned / plain["plain_seconds"]
+    return r
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=200, help="training graphs (seeds 1000..)")
-    ap.add_argument("--test", type=int, default=20, help="held-out graphs (seeds 5000..)")
-    ap.add_argument("--nodes", type=int, default=50_000)
-    ap.add_argument("--target-iters", type=int, default=1000)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--out", default=None)
-    args = ap.parse_args(argv)
+def rmat_engine(g):
+    """Engine for an R-MAT test graph with its stress pairs, and the reference's glibc start."""
+    e = Engine(g.n, g.src, g.dst, **FA2_RMAT)
+    e.set_graph_capture(True)
+    e.stress_setup(1024, 977, seed=0)
+    side = 2.0 * np.sqrt(g.n)
+    return e, graphs.glibc_random_positions(g.n, side, side, seed=1234)
 
-    def log(msg):
-        print(f"[learned] {msg}", file=sys.stderr, flush=True)
 
+def summarise(results):
+    ratios = [r["time_ratio"] for r in results if "time_ratio" in r]
+    return {"reached": len(ratios), "within_half": sum(1 for x in ratios if x <= 0.5),
+            "median_time_ratio": float(np.median(ratios)) if ratios else None,
+            "mean_time_ratio": float(np.mean(ratios)) if ratios else None}
+
+
+def run_sbm(args, log):
+    fs_list = [int(x) for x in args.sweep_feature_steps.split(",")] if args.validate else [args.feature_steps]
+    kappas = [float(x) for x in args.sweep_kappa.split(",")] if args.validate else [args.kappa]
     t0 = time.perf_counter()
     train_gs = [sbm_graph(1000 + k, n=args.nodes) for k in range(args.train)]
     log(f"{len(train_gs)} training graphs in {time.perf_counter() - t0:.1f}s")
     t1 = time.perf_counter()
     targets = fa2_targets(train_gs, args.target_iters, log=log)
     log(f"targets in {time.perf_counter() - t1:.1f}s")
-    t2 = time.perf_counter()
-    m = train(train_gs, targets, args.steps, lr=args.lr, log=log)
-    train_seconds = time.perf_counter() - t2
+    models, train_seconds = {}, {}
+    for fs in fs_list:
+        t2 = time.perf_counter()
+        models[fs] = train(train_gs, targets, args.steps, lr=args.lr, feature_steps=fs, log=log)
+        train_seconds[fs] = time.perf_counter() - t2
     del train_gs, targets
+    # validation graphs come from the training seed range past the graphs trained on
+    seed0 = 2000 if args.validate else 5000
     results = []
     for k in range(args.test):
-        g = sbm_graph(5000 + k, n=args.nodes)
-        r = evaluate(m, g, 5000 + k, iters_plain=args.target_iters)
-        results.append(r)
-        log(json.dumps(r))
-    ratios = [r["time_ratio"] for r in results if "time_ratio" in r]
-    summary = {"train_graphs": args.train, "test_graphs": args.test, "nodes": args.nodes,
+        seed = seed0 + k
+        g = sbm_graph(seed, n=args.nodes)
+        e = Engine(g.n, g.src, g.dst, **FA2)
+        e.set_graph_capture(True)
+        e.stress_setup(1024, 977, seed=seed)
+        plain = plain_run(e, start_positions(g, 10_000 + seed), args.target_iters)
+        for fs, m in models.items():
+            for kappa in kappas:
+                r = learned_run(m, e, plain, kappa * m.target_scale(), 10_000 + seed, fs, args.check)
+                r.update({"seed": seed, "nodes": g.n, "edges": g.m, "kappa": kappa})
+                results.append(r)
+                log(json.dumps({q: v for q, v in r.items() if q != "learned_trajectory"}))
+        e.close()
+    for m in models.values():
+        m.close()
+    summary = {"workload": "sbm50k", "validate": bool(args.validate), "train_graphs": args.train,
+               "test_graphs": args.test, "first_seed": seed0, "nodes": args.nodes,
                "target_iterations": args.target_iters, "train_steps": args.steps, "train_seconds": train_seconds,
-               "reached": len(ratios), "median_time_ratio": float(np.median(ratios)) if ratios else None,
-               "mean_time_ratio": float(np.mean(ratios)) if ratios else None, "results": results}
-    print(json.dumps(summary))
+               **summarise(results), "results": results}
+    if args.validate:
+        summary["by_setting"] = {f"{fs},{kappa}": summarise([r for r in results if r["feature_steps"] == fs
+                                                             and r["kappa"] == kappa])
+                                 for fs in fs_list for kappa in kappas}
+    return summary
+
+
+def run_rmat(args, log):
+    fs_list = [int(x) for x in args.sweep_feature_steps.split(",")] if args.validate else [args.feature_steps]
+    kappas = [float(x) for x in args.sweep_kappa.split(",")] if args.validate else [args.kappa]
+    t0 = time.perf_counter()
+    train_gs = [rmat_training_graph(k) for k in range(args.train)]
+    log(f"{len(train_gs)} training graphs in {time.perf_counter() - t0:.1f}s")
+    t1 = time.perf_counter()
+    targets = fa2_targets(train_gs, args.target_iters, params=FA2_RMAT, log=log)
+    log(f"targets in {time.perf_counter() - t1:.1f}s")
+    k_r = FA2_RMAT["scale"]
+    c = size_constant(train_gs, targets, k_r)
+    log(f"size constant c = {c:.4f}")
+    models, train_seconds = {}, {}
+    for fs in fs_list:
+        t2 = time.perf_counter()
+        models[fs] = train(train_gs, targets, args.steps, lr=args.lr, feature_steps=fs, log=log)
+        train_seconds[fs] = time.perf_counter() - t2
+    del train_gs, targets
+    if args.validate:    # the test's size and iteration budget, another seed
+        tests = [("rmat-scale24-128M-edges-seed4", 24, 128_000_000, 4, 200)]
+    else:
+        tests = [("rmat-scale24-128M-edges-seed3", 24, 128_000_000, 3, 200),
+                 ("rmat-scale20-10M-edges-seed1", 20, 10_000_000, 1, 500)]
+    results = []
+    for name, scale, edges, seed, iters in tests:
+        g = rmat_graph(scale, edges, seed)
+        log(f"{name}: V={g.n} E={g.m}")
+        e, start = rmat_engine(g)
+        plain = plain_run(e, start, iters)
+        log(f"plain: {plain}")
+        for fs, m in models.items():
+            for kappa in kappas:
+                r = learned_run(m, e, plain, float(kappa * c * np.sqrt(k_r * g.m)), 77, fs, args.check)
+                r.update({"graph": name, "nodes": g.n, "edges": g.m, "kappa": kappa})
+                results.append(r)
+                log(json.dumps({q: v for q, v in r.items() if q != "learned_trajectory"}))
+        e.close()
+        del g
+    for m in models.values():
+        m.close()
+    return {"workload": "rmat", "validate": bool(args.validate), "train_graphs": args.train,
+            "target_iterations": args.target_iters, "train_steps": args.steps, "train_seconds": train_seconds,
+            "size_constant": c, **summarise(results), "results": results}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=sorted(DEFAULTS), default="sbm50k")
+    ap.add_argument("--train", type=int, default=None, help="training graphs (seeds 1000..)")
+    ap.add_argument("--test", type=int, default=None, help="held-out SBM graphs (seeds 5000..)")
+    ap.add_argument("--nodes", type=int, default=50_000)
+    ap.add_argument("--target-iters", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--feature-steps", type=int, default=None)
+    ap.add_argument("--kappa", type=float, default=None)
+    ap.add_argument("--check", type=int, default=None, help="iterations between stress checks")
+    ap.add_argument("--validate", action="store_true", help="sweep settings on validation graphs, not the tests")
+    ap.add_argument("--sweep-feature-steps", default="0,1,2")
+    ap.add_argument("--sweep-kappa", default="0.25,0.5,1.0")
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    d = DEFAULTS[args.workload]
+    for key in ("feature_steps", "kappa", "train", "check", "test"):
+        if getattr(args, key) is None:
+            setattr(args, key, d[key])
+
+    def log(msg):
+        print(f"[learned] {msg}", file=sys.stderr, flush=True)
+
+    summary = run_sbm(args, log) if args.workload == "sbm50k" else run_rmat(args, log)
+    print(json.dumps({k: v for k, v in summary.items() if k != "results"}))
     if args.out:
         with open(args.out, "w") as fh:
             json.dump(summary, fh, indent=1)

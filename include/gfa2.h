@@ -244,6 +244,12 @@ int gfa2_mpnn_set_params(gfa2_mpnn_t m, const float* params);
  * NULL) is the layout to learn, interleaved x,y. */
 int gfa2_mpnn_add_graph(gfa2_mpnn_t m, int32_t n, const int64_t* rowptr, const int32_t* col, const float* target_xy,
                         uint64_t feature_seed, int32_t smoothing_steps, int32_t* graph_id);
+/* Registers the graph an engine holds (one graph, same device), copying its CSR device to
+ * device, and computes its features as gfa2_mpnn_add_graph does; no target. */
+int gfa2_mpnn_add_engine_graph(gfa2_mpnn_t m, gfa2_handle h, uint64_t feature_seed, int32_t smoothing_steps,
+                               int32_t* graph_id);
+/* Frees a registered graph's device memory; its id is invalid afterwards. */
+int gfa2_mpnn_release_graph(gfa2_mpnn_t m, int32_t graph_id);
 /* Features of a registered graph, [n x 16] (columns 9..15 are zero). */
 int gfa2_mpnn_features(gfa2_mpnn_t m, int32_t graph_id, float* X);
 /* Loss and its gradient (w.r.t. the BF16-rounded parameters) on one graph; pairs are the

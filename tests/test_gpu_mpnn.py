@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from gpugraphlayout_amd import Engine, Mpnn, graphs, learned
+from gpugraphlayout_amd import Engine, GFA2Error, Mpnn, graphs, learned
 from oracle import mpnn_oracle as mo
 
 pytestmark = pytest.mark.gpu
@@ -12,6 +12,18 @@ def graph_case(seed, n=600):
     s, t, _ = graphs.sbm(n, 6, 2.0, seed)
     g = graphs.edges_to_graph(s, t)
     rowptr, col = g.csr()
+    return g, rowptr, col
+
+
+def hub_case(seed, n=1500):
+    """An SBM graph plus two hubs adjacent to every other node: rows of about n entries take
+    the chunked aggregation path (rows over 512 entries, chunks of 512)."""
+    s, t, _ = graphs.sbm(n, 6, 2.0, seed)
+    hubs_s = np.concatenate([np.full(n, n), np.full(n + 1, n + 1)])
+    hubs_t = np.concatenate([np.arange(n), np.arange(n + 1)])
+    g = graphs.edges_to_graph(np.concatenate([s, hubs_s]), np.concatenate([t, hubs_t]))
+    rowptr, col = g.csr()
+    assert np.diff(rowptr).max() > 2 * 512
     return g, rowptr, col
 
 
@@ -28,6 +40,43 @@ def test_features_match_oracle():
     X = m.features(gid)
     ref = mo.features(rowptr, col, 1234, 8)
     assert np.max(np.abs(X - ref)) < 1e-4
+    m.close()
+
+
+def test_hub_rows_match_oracle():
+    """Features, loss and gradient on a graph with hub rows (chunked aggregation)."""
+    print("case mpnn hub rows", flush=True)
+    g, rowptr, col = hub_case(3)
+    X_ref = mo.features(rowptr, col, 4321, 3)
+    rng = np.random.default_rng(6)
+    T = rng.normal(size=(g.n, 2)).astype(np.float32)
+    m = Mpnn(seed=5)
+    gid = m.add_graph(g.n, rowptr, col, target=T, feature_seed=4321, steps=3)
+    X = m.features(gid)
+    assert np.max(np.abs(X - X_ref)) < 1e-4
+    loss, grad = m.loss_grad(gid, [5, 77])
+    L, G = mo.loss_grad(m.params().astype(np.float64), rowptr, col, X, normalised(T), [5, 77], round_bf16=True)
+    assert loss == pytest.approx(L, rel=1e-5)
+    assert np.linalg.norm(grad - G) / np.linalg.norm(G) < 1e-4
+    m.close()
+
+
+def test_engine_graph_features_match_host_graph_features():
+    """Features from an engine's device CSR equal those from the same CSR uploaded from the
+    host, bit for bit."""
+    print("case mpnn engine-graph features", flush=True)
+    g, rowptr, col = graph_case(4)
+    e = Engine(g.n, rowptr=rowptr, col=col)
+    m = Mpnn(seed=0)
+    a = m.add_graph(g.n, rowptr, col, feature_seed=77, steps=1)
+    b = m.add_engine_graph(e, feature_seed=77, steps=1)
+    assert np.array_equal(m.features(a), m.features(b))
+    assert np.array_equal(m.infer(a, 5.0), m.infer(b, 5.0))
+    m.release_graph(b)
+    with pytest.raises(GFA2Error):
+        m.features(b)
+    assert np.array_equal(m.features(a), m.features(m.add_engine_graph(e, feature_seed=77, steps=1)))
+    e.close()
     m.close()
 
 
