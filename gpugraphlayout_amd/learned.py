@@ -275,6 +275,30 @@ def run_rmat(args, log):
             "size_constant": c, **summarise(results), "results": results}
 
 
+def inference_timing(reps, log):
+    """Wall time of features (from the engine's CSR) and of inference into the engine on
+    R-MAT 2^24 with 128M edges (seed 3); the weights are untrained, which does not change the
+    work."""
+    g = rmat_graph(24, 128_000_000, 3)
+    log(f"rmat-scale24-128M-edges-seed3: V={g.n} E={g.m}")
+    e = Engine(g.n, g.src, g.dst, **FA2_RMAT)
+    m = Mpnn(seed=1)
+    out = []
+    for rep in range(reps):
+        e.sync()
+        t0 = time.perf_counter()
+        gid = m.add_engine_graph(e, feature_seed=77, steps=DEFAULTS["rmat"]["feature_steps"])
+        t1 = time.perf_counter()
+        m.infer_to_engine(gid, 1000.0, e)
+        t2 = time.perf_counter()
+        m.release_graph(gid)
+        out.append({"features_ms": 1e3 * (t1 - t0), "inference_ms": 1e3 * (t2 - t1)})
+        log(json.dumps(out[-1]))
+    m.close()
+    e.close()
+    return {"workload": "rmat", "inference_timing": out, "nodes": g.n, "edges": g.m}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=sorted(DEFAULTS), default="sbm50k")
@@ -290,6 +314,8 @@ def main(argv=None):
     ap.add_argument("--validate", action="store_true", help="sweep settings on validation graphs, not the tests")
     ap.add_argument("--sweep-feature-steps", default="0,1,2")
     ap.add_argument("--sweep-kappa", default="0.25,0.5,1.0")
+    ap.add_argument("--inference-timing", type=int, default=0,
+                    help="rmat: time features + inference on R-MAT 2^24 this many times, then exit")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     d = DEFAULTS[args.workload]
@@ -300,7 +326,10 @@ def main(argv=None):
     def log(msg):
         print(f"[learned] {msg}", file=sys.stderr, flush=True)
 
-    summary = run_sbm(args, log) if args.workload == "sbm50k" else run_rmat(args, log)
+    if args.inference_timing:
+        summary = inference_timing(args.inference_timing, log)
+    else:
+        summary = run_sbm(args, log) if args.workload == "sbm50k" else run_rmat(args, log)
     print(json.dumps({k: v for k, v in summary.items() if k != "results"}))
     if args.out:
         with open(args.out, "w") as fh:
