@@ -407,6 +407,12 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     const int nlight = (int)lists.size();
     lists.insert(lists.end(), medium.begin(), medium.end());
     lists.insert(lists.end(), heavy.begin(), heavy.end());
+    std::vector<int2> attr_chunks;
+    std::vector<int> attr_c0(1, 0);
+    for (size_t q = 0; q < heavy.size(); ++q) {
+        for (int e = rp32[heavy[q]]; e < (int)rowptr[heavy[q] + 1]; e += kAttrChunk) attr_chunks.push_back(make_int2((int)q, e));
+        attr_c0.push_back((int)attr_chunks.size());
+    }
 
     free_all(h);
     const int factor = h->user.cell_factor > 0 ? h->user.cell_factor : 3;
@@ -435,6 +441,12 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.rowptr, (size_t)n + 1);
     GFA2_ALLOC(b.col, (size_t)nnz);
     GFA2_ALLOC(b.node_lists, n);
+    GFA2_ALLOC(b.attr_chunks, attr_chunks.size());
+    GFA2_ALLOC(b.attr_c0, attr_c0.size());
+    GFA2_ALLOC(b.attr_part, attr_chunks.size());
+    GFA2_ALLOC(b.rank, n);
+    GFA2_ALLOC(b.col_rank, (size_t)nnz);
+    GFA2_ALLOC(b.pos_rank, n);
     GFA2_ALLOC(b.fga, n);
     GFA2_ALLOC(b.f, n);
     GFA2_ALLOC(b.fprev, n);
@@ -508,6 +520,24 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
         GFA2_TRY(hipMemcpyAsync(b.col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, h->stream));
     GFA2_TRY(hipMemcpyAsync(b.mass, m.data(), sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
     GFA2_TRY(hipMemcpyAsync(b.node_lists, lists.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+    b.nattr_chunks = (int)attr_chunks.size();
+    if (!attr_chunks.empty())
+        GFA2_TRY(hipMemcpyAsync(b.attr_chunks, attr_chunks.data(), sizeof(int2) * attr_chunks.size(), hipMemcpyHostToDevice,
+                                h->stream));
+    GFA2_TRY(hipMemcpyAsync(b.attr_c0, attr_c0.data(), sizeof(int) * attr_c0.size(), hipMemcpyHostToDevice, h->stream));
+    {
+        // rank = position in the order of decreasing degree (ties by id), by counting sort
+        int maxdeg = 0;
+        for (int v = 0; v < n; ++v) maxdeg = std::max(maxdeg, rp32[v + 1] - rp32[v]);
+        std::vector<int> start((size_t)maxdeg + 2, 0);
+        for (int v = 0; v < n; ++v) ++start[(size_t)(maxdeg - (rp32[v + 1] - rp32[v])) + 1];
+        for (int d = 0; d <= maxdeg; ++d) start[(size_t)d + 1] += start[(size_t)d];
+        std::vector<int> rank((size_t)n);
+        for (int v = 0; v < n; ++v) rank[v] = start[(size_t)(maxdeg - (rp32[v + 1] - rp32[v]))]++;
+        GFA2_TRY(hipMemcpyAsync(b.rank, rank.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+        launch_col_rank(b, h->stream);
+        GFA2_TRY(hipStreamSynchronize(h->stream));   // rank goes out of scope
+    }
     GFA2_TRY(hipMemcpyAsync(b.node_off, node_off, sizeof(int) * (nseg + 1), hipMemcpyHostToDevice, h->stream));
     GFA2_TRY(hipMemsetAsync(b.pos, 0, sizeof(float2) * n, h->stream));
     GFA2_TRY(hipMemsetAsync(b.rec_begin, 0, sizeof(int) * (nseg + 1), h->stream));

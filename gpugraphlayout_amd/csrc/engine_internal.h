@@ -99,6 +99,8 @@ struct LaunchConfig {
     int walk_fetch;         // 0: one record load per step; 1: group-wide windows; 2: records in pairs
 };
 
+constexpr int kAttrChunk = 4096;    // entries per block for rows over 2,048 entries
+
 // All device buffers of one instance.
 struct Bufs {
     int n;                  // V
@@ -111,6 +113,18 @@ struct Bufs {
     int* col;
     int* node_lists;        // light | medium | heavy node ids
     int nlight, nmedium, nheavy;
+    // Heavy rows are split into chunks of kAttrChunk entries, one block each; a second kernel
+    // adds each heavy row's chunk partials in chunk order.
+    int nattr_chunks;
+    int2* attr_chunks;      // {heavy index h, first entry}
+    int* attr_c0;           // [nheavy + 1] first chunk of each heavy row
+    float2* attr_part;      // [nattr_chunks]
+    // Attraction gathers neighbour positions from a copy ordered by decreasing degree: the
+    // positions most edges point at share few cache lines. pos_rank[rank[v]] = pos[v] and
+    // col_rank[e] = rank[col[e]], so pos_rank[col_rank[e]] is pos[col[e]], bit for bit.
+    int* rank;
+    int* col_rank;
+    float2* pos_rank;
     float2* fga;
     float2* f;
     float2* fprev;
@@ -184,6 +198,7 @@ void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipSt
 
 // Launch groups of one iteration, in order.
 void launch_attraction(const Bufs& b, const Params& p, hipStream_t s);
+void launch_col_rank(const Bufs& b, hipStream_t s);
 void launch_bbox_partials(const Bufs& b, hipStream_t s);
 void launch_keys(const Bufs& b, hipStream_t s);
 void launch_tree(Bufs& b, hipStream_t s);              // after the sort: cells + summaries

@@ -1,6 +1,8 @@
 // ForceAtlas2 force, speed and displacement kernels for gfx950 (FP32 VALU throughout).
 #include "engine_internal.h"
 
+#include <algorithm>
+
 namespace gfa2 {
 namespace {
 
@@ -49,12 +51,35 @@ __device__ __forceinline__ void attract_term(float2 p, float2 q, int linlog, con
     }
 }
 
+// col_rank[e] = rank[col[e]] (once per graph).
+__global__ __launch_bounds__(256) void k_gather_rank(const int* __restrict__ col, const int* __restrict__ rank,
+                                                     long long nnz, int* __restrict__ col_rank) {
+    for (long long e = blockIdx.x * 256ll + threadIdx.x; e < nnz; e += (long long)gridDim.x * 256)
+        col_rank[e] = rank[col[e]];
+}
+
+// pos_rank[rank[v]] = pos[v] (every iteration, before the attraction).
+__global__ __launch_bounds__(256) void k_rank_pos(const float2* __restrict__ pos, const int* __restrict__ rank, int n,
+                                                  float2* __restrict__ pos_rank) {
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v < n) pos_rank[rank[v]] = pos[v];
+}
+
+// Attraction + gravity, fga[v] = gravity(v) + sum over CSR entries of v (in entry order for the
+// light rows; lane-strided then a shuffle tree for medium rows; see below for heavy rows).
+// Degree classes: light rows (<= 32 entries) one thread each, entries four at a time;
+// medium rows (<= 2,048) one wave each; heavy rows in chunks of kAttrChunk entries, one block
+// per chunk writing its partial (thread-strided, then an LDS tree), added per row in chunk order
+// by k_attract_heavy. Neighbour positions come from the degree-ranked copy.
 template <bool GEN>
 __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos, const float* __restrict__ mass,
+                                                 const float2* __restrict__ pos_rank,
                                                  const int* __restrict__ rowptr, const int* __restrict__ col,
                                                  const int* __restrict__ lists, int nlight, int nmedium,
-                                                 int nheavy, int light_blocks, int medium_blocks,
-                                                 int strong, float k_g, float2* __restrict__ fga, int linlog,
+                                                 const int2* __restrict__ chunks, int nchunks,
+                                                 int light_blocks, int medium_blocks,
+                                                 int strong, float k_g, float2* __restrict__ fga,
+                                                 float2* __restrict__ part, int linlog,
                                                  const float* __restrict__ ew) {
     __shared__ float2 red[256];
     const int tid = threadIdx.x;
@@ -66,9 +91,16 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         const float2 p = pos[v];
         float ax = 0.0f, ay = 0.0f;
         const int e1 = rowptr[v + 1];
-        for (int e = rowptr[v]; e < e1; ++e) {
-            attract_term<GEN>(p, pos[col[e]], linlog, ew, e, ax, ay);
+        int e = rowptr[v];
+        for (; e + 4 <= e1; e += 4) {
+            const int j0 = col[e], j1 = col[e + 1], j2 = col[e + 2], j3 = col[e + 3];
+            const float2 q0 = pos_rank[j0], q1 = pos_rank[j1], q2 = pos_rank[j2], q3 = pos_rank[j3];
+            attract_term<GEN>(p, q0, linlog, ew, e, ax, ay);
+            attract_term<GEN>(p, q1, linlog, ew, e + 1, ax, ay);
+            attract_term<GEN>(p, q2, linlog, ew, e + 2, ax, ay);
+            attract_term<GEN>(p, q3, linlog, ew, e + 3, ax, ay);
         }
+        for (; e < e1; ++e) attract_term<GEN>(p, pos_rank[col[e]], linlog, ew, e, ax, ay);
         const float2 g = gravity(p, mass[v], strong, k_g);
         fga[v] = make_float2(g.x + ax, g.y + ay);
         return;
@@ -83,7 +115,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         float ax = 0.0f, ay = 0.0f;
         const int e1 = rowptr[v + 1];
         for (int e = rowptr[v] + lane; e < e1; e += 64) {
-            attract_term<GEN>(p, pos[col[e]], linlog, ew, e, ax, ay);
+            attract_term<GEN>(p, pos_rank[col[e]], linlog, ew, e, ax, ay);
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
@@ -97,14 +129,22 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         return;
     }
     blk -= medium_blocks;
-    if (blk >= nheavy) return;
-    const int v = lists[nlight + nmedium + blk];
+    if (blk >= nchunks) return;
+    const int2 ch = chunks[blk];
+    const int v = lists[nlight + nmedium + ch.x];
     const float2 p = pos[v];
     float ax = 0.0f, ay = 0.0f;
-    const int e1 = rowptr[v + 1];
-    for (int e = rowptr[v] + tid; e < e1; e += 256) {
-        attract_term<GEN>(p, pos[col[e]], linlog, ew, e, ax, ay);
+    const int e1 = min(ch.y + kAttrChunk, rowptr[v + 1]);
+    int e = ch.y + tid;
+    for (; e + 3 * 256 < e1; e += 4 * 256) {
+        const int j0 = col[e], j1 = col[e + 256], j2 = col[e + 512], j3 = col[e + 768];
+        const float2 q0 = pos_rank[j0], q1 = pos_rank[j1], q2 = pos_rank[j2], q3 = pos_rank[j3];
+        attract_term<GEN>(p, q0, linlog, ew, e, ax, ay);
+        attract_term<GEN>(p, q1, linlog, ew, e + 256, ax, ay);
+        attract_term<GEN>(p, q2, linlog, ew, e + 512, ax, ay);
+        attract_term<GEN>(p, q3, linlog, ew, e + 768, ax, ay);
     }
+    for (; e < e1; e += 256) attract_term<GEN>(p, pos_rank[col[e]], linlog, ew, e, ax, ay);
     red[tid] = make_float2(ax, ay);
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -114,10 +154,24 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        const float2 g = gravity(p, mass[v], strong, k_g);
-        fga[v] = make_float2(g.x + red[0].x, g.y + red[0].y);
+    if (tid == 0) part[blk] = red[0];
+}
+
+// Heavy rows: gravity + the row's chunk partials in chunk order.
+__global__ __launch_bounds__(256) void k_attract_heavy(const float2* __restrict__ pos, const float* __restrict__ mass,
+                                                       const int* __restrict__ heavy, int nheavy,
+                                                       const int* __restrict__ c0, const float2* __restrict__ part,
+                                                       int strong, float k_g, float2* __restrict__ fga) {
+    const int h = blockIdx.x * 256 + threadIdx.x;
+    if (h >= nheavy) return;
+    const int v = heavy[h];
+    float ax = 0.0f, ay = 0.0f;
+    for (int c = c0[h]; c < c0[h + 1]; ++c) {
+        ax += part[c].x;
+        ay += part[c].y;
     }
+    const float2 g = gravity(pos[v], mass[v], strong, k_g);
+    fga[v] = make_float2(g.x + ax, g.y + ay);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -578,17 +632,29 @@ void walk_dispatch(const Bufs& b, const Params& p, hipStream_t s) {
 
 }  // namespace
 
+void launch_col_rank(const Bufs& b, hipStream_t s) {
+    if (b.nnz > 0) k_gather_rank<<<(int)std::min<long long>((b.nnz + 255) / 256, 16384), 256, 0, s>>>(b.col, b.rank, b.nnz, b.col_rank);
+}
+
 void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
+    k_rank_pos<<<(b.n + 255) / 256, 256, 0, s>>>(b.pos, b.rank, b.n, b.pos_rank);
     const int lb = (b.nlight + 255) / 256;
     const int mb = (b.nmedium + 3) / 4;
-    const int total = lb + mb + b.nheavy;
-    if (total == 0) return;
-    if (p.linlog || b.ew)
-        k_attract<true><<<total, 256, 0, s>>>(b.pos, b.mass, b.rowptr, b.col, b.node_lists, b.nlight, b.nmedium,
-                                              b.nheavy, lb, mb, p.strong_gravity, p.k_g, b.fga, p.linlog, b.ew);
-    else
-        k_attract<false><<<total, 256, 0, s>>>(b.pos, b.mass, b.rowptr, b.col, b.node_lists, b.nlight, b.nmedium,
-                                               b.nheavy, lb, mb, p.strong_gravity, p.k_g, b.fga, 0, nullptr);
+    const int total = lb + mb + b.nattr_chunks;
+    if (total > 0) {
+        if (p.linlog || b.ew)
+            k_attract<true><<<total, 256, 0, s>>>(b.pos, b.mass, b.pos_rank, b.rowptr, b.col_rank, b.node_lists, b.nlight,
+                                                  b.nmedium, b.attr_chunks, b.nattr_chunks, lb, mb, p.strong_gravity,
+                                                  p.k_g, b.fga, b.attr_part, p.linlog, b.ew);
+        else
+            k_attract<false><<<total, 256, 0, s>>>(b.pos, b.mass, b.pos_rank, b.rowptr, b.col_rank, b.node_lists,
+                                                   b.nlight, b.nmedium, b.attr_chunks, b.nattr_chunks, lb, mb,
+                                                   p.strong_gravity, p.k_g, b.fga, b.attr_part, 0, nullptr);
+    }
+    if (b.nheavy > 0)
+        k_attract_heavy<<<(b.nheavy + 255) / 256, 256, 0, s>>>(b.pos, b.mass, b.node_lists + b.nlight + b.nmedium,
+                                                               b.nheavy, b.attr_c0, b.attr_part, p.strong_gravity,
+                                                               p.k_g, b.fga);
 }
 
 void launch_walk(const Bufs& b, const Params& p, hipStream_t s) {
