@@ -114,9 +114,16 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         const float2 p = pos[v];
         float ax = 0.0f, ay = 0.0f;
         const int e1 = rowptr[v + 1];
-        for (int e = rowptr[v] + lane; e < e1; e += 64) {
-            attract_term<GEN>(p, pos_rank[col[e]], linlog, ew, e, ax, ay);
+        int e = rowptr[v] + lane;
+        for (; e + 192 < e1; e += 256) {
+            const int j0 = col[e], j1 = col[e + 64], j2 = col[e + 128], j3 = col[e + 192];
+            const float2 q0 = pos_rank[j0], q1 = pos_rank[j1], q2 = pos_rank[j2], q3 = pos_rank[j3];
+            attract_term<GEN>(p, q0, linlog, ew, e, ax, ay);
+            attract_term<GEN>(p, q1, linlog, ew, e + 64, ax, ay);
+            attract_term<GEN>(p, q2, linlog, ew, e + 128, ax, ay);
+            attract_term<GEN>(p, q3, linlog, ew, e + 192, ax, ay);
         }
+        for (; e < e1; e += 64) attract_term<GEN>(p, pos_rank[col[e]], linlog, ew, e, ax, ay);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             ax += __shfl_xor(ax, o, 64);
