@@ -72,22 +72,58 @@ __global__ __launch_bounds__(256) void k_rowscan(unsigned int* __restrict__ hist
     if (lane == 63) rowtot[row] = inc;
 }
 
+// Stable ranks within a tile, three block barriers per tile: each wave owns a contiguous
+// quarter of the tile (kSortItems rows of 64 pairs), ranks its pairs row by row with
+// a wave64 ballot match against its own running digit counts in LDS, then one pass over the
+// digits turns the per-wave counts into destinations (wave order is tile order, so the sort
+// stays stable).
+__device__ __forceinline__ void rank_rows(const unsigned long long (&key)[kSortItems], int wbase, int n, int shift,
+                                          unsigned int* __restrict__ wcnt, unsigned int (&rank)[kSortItems]) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const bool valid = wbase + r * 64 + lane < n;
+        const unsigned int d = (unsigned)(key[r] >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const unsigned int rr = (unsigned)__popcll(peers & lt);
+        // every lane reads the digit's count before its leader writes it (one wave, LDS in order)
+        const unsigned int before = valid ? wcnt[d] : 0u;
+        rank[r] = before + rr;
+        if (valid && rr == 0) wcnt[d] = before + (unsigned)__popcll(peers);
+    }
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long long* __restrict__ kin,
                                                           const unsigned int* __restrict__ vin,
                                                           unsigned long long* __restrict__ kout,
                                                           unsigned int* __restrict__ vout, int n, int shift,
                                                           const unsigned int* __restrict__ hist, int tiles,
                                                           const unsigned int* __restrict__ rowtot) {
-    __shared__ unsigned int dstart[256];   // global start of each digit for this tile
     __shared__ unsigned int wcnt[kWaves][256];
     __shared__ unsigned int scan_tmp[256];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-
+    const int wbase = blockIdx.x * kSortTile + wave * (kSortItems * 64);
+    unsigned long long key[kSortItems];
+    unsigned int val[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int e = wbase + r * 64 + lane;
+        key[r] = e < n ? kin[e] : 0ull;
+        val[r] = e < n ? vin[e] : 0u;
+    }
     // Exclusive scan of digit totals (256 entries, one per thread), Hillis-Steele in LDS.
-    unsigned int tot = rowtot[tid];
+    const unsigned int tot = rowtot[tid];
     scan_tmp[tid] = tot;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
     __syncthreads();
     for (int o = 1; o < 256; o <<= 1) {
         unsigned int v = tid >= o ? scan_tmp[tid - o] : 0u;
@@ -95,49 +131,27 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long lo
         scan_tmp[tid] += v;
         __syncthreads();
     }
-    dstart[tid] = scan_tmp[tid] - tot + hist[(size_t)tid * tiles + blockIdx.x];
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+    const unsigned int dstart = scan_tmp[tid] - tot + hist[(size_t)tid * tiles + blockIdx.x];
+    unsigned int rank[kSortItems];
+    rank_rows(key, wbase, n, shift, wcnt[wave], rank);
     __syncthreads();
-
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int base = blockIdx.x * kSortTile;
+    {
+        unsigned int run = dstart;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const unsigned int c = wcnt[w][tid];
+            wcnt[w][tid] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int e = base + r * kSortThreads + tid;
-        const bool valid = e < n;
-        unsigned long long key = valid ? kin[e] : 0ull;
-        unsigned int val = valid ? vin[e] : 0u;
-        unsigned int d = (unsigned)(key >> shift) & 255u;
-        unsigned long long peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            unsigned long long m = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? m : ~m;
+        if (wbase + r * 64 + lane < n) {
+            const unsigned int dst = wcnt[wave][(unsigned)(key[r] >> shift) & 255u] + rank[r];
+            kout[dst] = key[r];
+            vout[dst] = val[r];
         }
-        const unsigned int rank = (unsigned)__popcll(peers & lt);
-        if (valid && rank == 0) wcnt[wave][d] = (unsigned)__popcll(peers);
-        __syncthreads();
-        // per digit: offsets of each wave's run, then advance the digit's start
-        {
-            unsigned int run = dstart[tid];
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                unsigned int c = wcnt[w][tid];
-                wcnt[w][tid] = run;
-                run += c;
-            }
-            dstart[tid] = run;
-        }
-        __syncthreads();
-        if (valid) {
-            unsigned int dst = wcnt[wave][d] + rank;
-            kout[dst] = key;
-            vout[dst] = val;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
-        __syncthreads();
     }
 }
 
@@ -176,7 +190,6 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
                                                            unsigned int* __restrict__ status,
                                                            unsigned long long* __restrict__ ticket, int tiles,
                                                            int* __restrict__ err) {
-    __shared__ unsigned int dstart[256];
     __shared__ unsigned int wcnt[kWaves][256];
     __shared__ unsigned int scan_tmp[256];
     __shared__ unsigned int tcount[256];
@@ -204,13 +217,13 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
     const unsigned int gstart = scan_tmp[tid] - tot;
     const int tile = s_tile;
     const unsigned int epoch = s_epoch << 27;
-    // the tile's pairs, held in registers, and its digit counts
-    const int base = tile * kSortTile;
+    // the tile's pairs, held in registers (each wave a contiguous quarter), and its digit counts
+    const int wbase = tile * kSortTile + wave * (kSortItems * 64);
     unsigned long long key[kSortItems];
     unsigned int val[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int e = base + r * kSortThreads + tid;
+        const int e = wbase + r * 64 + lane;
         key[r] = e < n ? kin[e] : 0ull;
         val[r] = e < n ? vin[e] : 0u;
         if (e < n) atomicAdd(&tcount[(unsigned)(key[r] >> shift) & 255u], 1u);
@@ -244,46 +257,30 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
         __hip_atomic_store(my, kStatPre | epoch | ((excl + cnt) & kStatCount), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    dstart[tid] = gstart + excl;
+    const unsigned int dstart = gstart + excl;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
     __syncthreads();
-    // stable ranks within the tile, row by row (the fallback's scatter, from registers)
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned int rank[kSortItems];
+    rank_rows(key, wbase, n, shift, wcnt[wave], rank);
+    __syncthreads();
+    {
+        unsigned int run = dstart;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const unsigned int c = wcnt[w][tid];
+            wcnt[w][tid] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int e = base + r * kSortThreads + tid;
-        const bool valid = e < n;
-        const unsigned int d = (unsigned)(key[r] >> shift) & 255u;
-        unsigned long long peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            unsigned long long m = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? m : ~m;
-        }
-        const unsigned int rank = (unsigned)__popcll(peers & lt);
-        if (valid && rank == 0) wcnt[wave][d] = (unsigned)__popcll(peers);
-        __syncthreads();
-        {
-            unsigned int run = dstart[tid];
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                unsigned int c = wcnt[w][tid];
-                wcnt[w][tid] = run;
-                run += c;
-            }
-            dstart[tid] = run;
-        }
-        __syncthreads();
-        if (valid) {
-            const unsigned int dst = wcnt[wave][d] + rank;
+        if (wbase + r * 64 + lane < n) {
+            const unsigned int dst = wcnt[wave][(unsigned)(key[r] >> shift) & 255u] + rank[r];
             kout[dst] = key[r];
             vout[dst] = val[r];
         }
-        __syncthreads();
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
-        __syncthreads();
     }
 }
 

@@ -121,3 +121,33 @@ def test_attraction_options_match_oracle(oracle, linlog, delta, weighted):
                                          f=oracle.gravity(pos, g.mass(), False, 1.0))
         assert rel_l2(eng.forces(GFA2_FORCE_ATTRACTION_GRAVITY), want) <= 1e-5
     eng.close()
+
+
+@pytest.mark.parametrize("linlog,weighted", [(0, False), (1, True)])
+def test_attraction_on_hub_rows_matches_oracle(oracle, linlog, weighted):
+    """Rows over 2,048 entries (attraction in 4,096-entry chunks, partials added in chunk
+    order) against the oracle, with and without linlog and weights; the hubs' own forces
+    are checked one by one."""
+    print(f"case attraction hub rows linlog={linlog} weighted={weighted}", flush=True)
+    n = 6000
+    s, t = graphs.barabasi_albert(n, 2, 8)
+    hs = np.concatenate([np.full(n, n), np.full(n + 1, n + 1)])
+    ht = np.concatenate([np.arange(n), np.arange(n + 1)])
+    g = graphs.edges_to_graph(np.concatenate([s, hs]), np.concatenate([t, ht]))
+    deg = np.diff(g.csr()[0])
+    hubs = np.nonzero(deg > 4096)[0]
+    assert len(hubs) == 2
+    pos = graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 12)
+    rng = np.random.default_rng(10)
+    w = rng.uniform(0.25, 4.0, g.m).astype(np.float32) if weighted else None
+    eng = Engine(g.n, g.src, g.dst, weights=w, scale=2.0, linlog=linlog, edge_weight_influence=1.0)
+    eng.set_positions(pos)
+    eng.step(1)
+    eng.sync()
+    fga = eng.forces(GFA2_FORCE_ATTRACTION_GRAVITY)
+    want = oracle.gravity(pos, g.mass(), False, 1.0)
+    want = oracle.attraction_general(pos, g.src, g.dst, w=w, delta=1.0, linlog=bool(linlog), f=want)
+    assert rel_l2(fga, want) <= 1e-5
+    for h in hubs:
+        assert rel_l2(fga[h], want[h]) <= 1e-5
+    eng.close()
