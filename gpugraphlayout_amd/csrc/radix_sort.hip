@@ -99,14 +99,94 @@ __device__ __forceinline__ void rank_rows(const unsigned long long (&key)[kSortI
     }
 }
 
+// Exclusive scan of one value per thread over the block's 256 threads: shuffle scans within
+// the waves, then the four wave totals (one barrier). sh: 4 words of LDS, not reused before
+// another barrier.
+__device__ __forceinline__ unsigned int block_exscan256(unsigned int v, unsigned int* __restrict__ sh) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    unsigned int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    unsigned int off = 0u;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+        if (w < wave) off += sh[w];
+    return off + inc - v;
+}
+
+// LDS of the tile step: per-wave digit counts, the staged tile, scan scratch.
+struct TileLds {
+    unsigned int wcnt[kWaves][256];
+    unsigned long long skey[kSortTile];
+    unsigned int sval[kSortTile];
+    int delta[256];                 // global destination minus tile position, per digit
+    unsigned int sh[2][kWaves];
+};
+
+// Writes one tile's pairs (held in registers, each wave a contiguous quarter) to their sorted
+// places. dglobal: this thread's digit's first destination for the tile. The pairs are ranked
+// stably (rank_rows), placed in LDS in digit order, then written out in tile order, so the
+// pairs of a digit leave as one run of consecutive addresses.
+__device__ __forceinline__ void scatter_tile(const unsigned long long (&key)[kSortItems],
+                                             const unsigned int (&val)[kSortItems], int tile, int n, int shift,
+                                             unsigned int dglobal, TileLds& L,
+                                             unsigned long long* __restrict__ kout, unsigned int* __restrict__ vout) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wbase = tile * kSortTile + wave * (kSortItems * 64);
+    unsigned int rank[kSortItems];
+    rank_rows(key, wbase, n, shift, L.wcnt[wave], rank);
+    __syncthreads();
+    // per digit (one per thread): the tile's count, its start within the tile, each wave's start
+    unsigned int c[kWaves], tcount = 0u;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        c[w] = L.wcnt[w][tid];
+        tcount += c[w];
+    }
+    const unsigned int tstart = block_exscan256(tcount, L.sh[1]);
+    {
+        unsigned int run = tstart;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            L.wcnt[w][tid] = run;
+            run += c[w];
+        }
+    }
+    L.delta[tid] = (int)dglobal - (int)tstart;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        if (wbase + r * 64 + lane < n) {
+            const unsigned int at = L.wcnt[wave][(unsigned)(key[r] >> shift) & 255u] + rank[r];
+            L.skey[at] = key[r];
+            L.sval[at] = val[r];
+        }
+    }
+    __syncthreads();
+    const int count = min(kSortTile, n - tile * kSortTile);
+    for (int j = tid; j < count; j += kSortThreads) {
+        const unsigned long long k = L.skey[j];
+        const unsigned int dst = (unsigned int)(L.delta[(unsigned)(k >> shift) & 255u] + j);
+        kout[dst] = k;
+        vout[dst] = L.sval[j];
+    }
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long long* __restrict__ kin,
                                                           const unsigned int* __restrict__ vin,
                                                           unsigned long long* __restrict__ kout,
                                                           unsigned int* __restrict__ vout, int n, int shift,
                                                           const unsigned int* __restrict__ hist, int tiles,
                                                           const unsigned int* __restrict__ rowtot) {
-    __shared__ unsigned int wcnt[kWaves][256];
-    __shared__ unsigned int scan_tmp[256];
+    __shared__ TileLds L;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -119,40 +199,11 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long lo
         key[r] = e < n ? kin[e] : 0ull;
         val[r] = e < n ? vin[e] : 0u;
     }
-    // Exclusive scan of digit totals (256 entries, one per thread), Hillis-Steele in LDS.
-    const unsigned int tot = rowtot[tid];
-    scan_tmp[tid] = tot;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        unsigned int v = tid >= o ? scan_tmp[tid - o] : 0u;
-        __syncthreads();
-        scan_tmp[tid] += v;
-        __syncthreads();
-    }
-    const unsigned int dstart = scan_tmp[tid] - tot + hist[(size_t)tid * tiles + blockIdx.x];
-    unsigned int rank[kSortItems];
-    rank_rows(key, wbase, n, shift, wcnt[wave], rank);
-    __syncthreads();
-    {
-        unsigned int run = dstart;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const unsigned int c = wcnt[w][tid];
-            wcnt[w][tid] = run;
-            run += c;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        if (wbase + r * 64 + lane < n) {
-            const unsigned int dst = wcnt[wave][(unsigned)(key[r] >> shift) & 255u] + rank[r];
-            kout[dst] = key[r];
-            vout[dst] = val[r];
-        }
-    }
+    for (int w = 0; w < kWaves; ++w) L.wcnt[w][tid] = 0;
+    // digit starts: exclusive scan of the digit totals, plus this tile's offset within the digit
+    const unsigned int dglobal = block_exscan256(rowtot[tid], L.sh[0]) + hist[(size_t)tid * tiles + blockIdx.x];
+    scatter_tile(key, val, blockIdx.x, n, shift, dglobal, L, kout, vout);
 }
 
 // ---- onesweep -------------------------------------------------------------------------------
@@ -190,8 +241,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
                                                            unsigned int* __restrict__ status,
                                                            unsigned long long* __restrict__ ticket, int tiles,
                                                            int* __restrict__ err) {
-    __shared__ unsigned int wcnt[kWaves][256];
-    __shared__ unsigned int scan_tmp[256];
+    __shared__ TileLds L;
     __shared__ unsigned int tcount[256];
     __shared__ int s_tile;
     __shared__ unsigned int s_epoch;
@@ -204,17 +254,10 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
         s_epoch = (unsigned int)((t / (unsigned long long)tiles) & 7ull);
     }
     tcount[tid] = 0u;
-    // exclusive scan of this pass's digit totals (Hillis-Steele in LDS)
-    const unsigned int tot = digit_hist[tid];
-    scan_tmp[tid] = tot;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        unsigned int v = tid >= o ? scan_tmp[tid - o] : 0u;
-        __syncthreads();
-        scan_tmp[tid] += v;
-        __syncthreads();
-    }
-    const unsigned int gstart = scan_tmp[tid] - tot;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) L.wcnt[w][tid] = 0;
+    // exclusive scan of this pass's digit totals (its barrier also publishes s_tile and the zeros)
+    const unsigned int gstart = block_exscan256(digit_hist[tid], L.sh[0]);
     const int tile = s_tile;
     const unsigned int epoch = s_epoch << 27;
     // the tile's pairs, held in registers (each wave a contiguous quarter), and its digit counts
@@ -257,31 +300,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
         __hip_atomic_store(my, kStatPre | epoch | ((excl + cnt) & kStatCount), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    const unsigned int dstart = gstart + excl;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
-    __syncthreads();
-    unsigned int rank[kSortItems];
-    rank_rows(key, wbase, n, shift, wcnt[wave], rank);
-    __syncthreads();
-    {
-        unsigned int run = dstart;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const unsigned int c = wcnt[w][tid];
-            wcnt[w][tid] = run;
-            run += c;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        if (wbase + r * 64 + lane < n) {
-            const unsigned int dst = wcnt[wave][(unsigned)(key[r] >> shift) & 255u] + rank[r];
-            kout[dst] = key[r];
-            vout[dst] = val[r];
-        }
-    }
+    scatter_tile(key, val, tile, n, shift, gstart + excl, L, kout, vout);
 }
 
 }  // namespace
