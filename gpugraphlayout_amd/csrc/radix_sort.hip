@@ -39,37 +39,53 @@ __global__ __launch_bounds__(kSortThreads) void k_upsweep(const unsigned long lo
     hist[tid * tiles + blockIdx.x] = h[tid];
 }
 
-// One wave per digit row: exclusive scan over tiles, row total into rowtot.
+// One block per digit row: exclusive scan over the tiles, row total into rowtot. The row is
+// read in chunks of 256 tiles (one per thread, coalesced), up to kRowChunks chunks requested
+// before any is scanned; each chunk is one block scan (shuffles within the waves, then the
+// four wave totals) carried into the next.
+constexpr int kRowChunks = 16;
+
 __global__ __launch_bounds__(256) void k_rowscan(unsigned int* __restrict__ hist, int tiles,
                                                  unsigned int* __restrict__ rowtot) {
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= 256) return;
-    unsigned int* h = hist + (size_t)row * tiles;
-    const int per = (tiles + 63) / 64;
-    const int b0 = lane * per;
-    unsigned int sum = 0;
-    for (int k = 0; k < per; ++k) {
-        int t = b0 + k;
-        if (t < tiles) sum += h[t];
-    }
-    // inclusive wave scan of per-lane sums
-    unsigned int inc = sum;
+    __shared__ unsigned int sh[2][4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    unsigned int* h = hist + (size_t)blockIdx.x * tiles;
+    unsigned int carry = 0u;
+    int parity = 0;
+    for (int c0 = 0; c0 < tiles; c0 += 256 * kRowChunks) {
+        unsigned int v[kRowChunks];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        unsigned int v = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += v;
-    }
-    unsigned int run = inc - sum;
-    for (int k = 0; k < per; ++k) {
-        int t = b0 + k;
-        if (t < tiles) {
-            unsigned int c = h[t];
-            h[t] = run;
-            run += c;
+        for (int c = 0; c < kRowChunks; ++c) {
+            const int t = c0 + c * 256 + tid;
+            v[c] = t < tiles ? h[t] : 0u;
+        }
+#pragma unroll
+        for (int c = 0; c < kRowChunks; ++c) {
+            if (c0 + c * 256 >= tiles) break;            // uniform
+            unsigned int inc = v[c];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned int t = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += t;
+            }
+            if (lane == 63) sh[parity][wave] = inc;
+            __syncthreads();
+            unsigned int off = 0u, total = 0u;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const unsigned int x = sh[parity][w];
+                if (w < wave) off += x;
+                total += x;
+            }
+            const int t = c0 + c * 256 + tid;
+            if (t < tiles) h[t] = carry + off + inc - v[c];
+            carry += total;
+            parity ^= 1;
         }
     }
-    if (lane == 63) rowtot[row] = inc;
+    if (tid == 0) rowtot[blockIdx.x] = carry;
 }
 
 // Stable ranks within a tile, three block barriers per tile: each wave owns a contiguous
@@ -327,7 +343,7 @@ void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipSt
     } else {
         for (int shift = begin_bit; shift < end_bit; shift += 8) {
             k_upsweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], n, shift, buf.hist, tiles);
-            k_rowscan<<<64, 256, 0, s>>>(buf.hist, tiles, buf.rowtot);
+            k_rowscan<<<256, 256, 0, s>>>(buf.hist, tiles, buf.rowtot);
             k_scatter<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], buf.vals[cur], buf.keys[cur ^ 1],
                                                      buf.vals[cur ^ 1], n, shift, buf.hist, tiles, buf.rowtot);
             cur ^= 1;
