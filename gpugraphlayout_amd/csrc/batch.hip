@@ -11,6 +11,8 @@
 // out in a batch follows exactly the trajectory it follows alone.
 #include "engine_internal.h"
 
+#include <algorithm>
+
 namespace gfa2 {
 namespace {
 
@@ -76,25 +78,55 @@ __global__ __launch_bounds__(256) void k_seg_keys(const float2* __restrict__ pos
     }
 }
 
+// The graph owning node v: the last g with node_off[g] <= v, by binary search over the node
+// offsets staged in LDS (up to kSegLds graphs; beyond, over global memory), instead of a
+// gather from the per-node graph table, which after the path sort is random over all nodes.
+constexpr int kSegLds = 8192;
+
+__device__ __forceinline__ int graph_of(const int* __restrict__ off, int nseg, unsigned int v) {
+    int lo = 0, hi = nseg;                  // off[lo] <= v < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((unsigned int)off[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ const int* stage_offsets(const int* __restrict__ node_off, int nseg, int* lds) {
+    if (nseg + 1 > kSegLds) return node_off;
+    for (int i = threadIdx.x; i <= nseg; i += blockDim.x) lds[i] = node_off[i];
+    __syncthreads();
+    return lds;
+}
+
 __global__ __launch_bounds__(256) void k_seg_gkeys(const unsigned int* __restrict__ vals, int n,
-                                                   const int* __restrict__ seg_of, unsigned long long* __restrict__ keys) {
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) keys[k] = (unsigned long long)seg_of[vals[k]];
+                                                   const int* __restrict__ node_off, int nseg,
+                                                   unsigned long long* __restrict__ keys) {
+    __shared__ int off_lds[kSegLds];
+    const int* off = stage_offsets(node_off, nseg, off_lds);
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256)
+        keys[k] = (unsigned long long)graph_of(off, nseg, vals[k]);
 }
 
 __global__ __launch_bounds__(256) void k_seg_regather(const unsigned int* __restrict__ vals, int n,
-                                                      const int* __restrict__ seg_of,
+                                                      const int* __restrict__ node_off, int nseg,
                                                       const unsigned long long* __restrict__ pkey,
                                                       unsigned long long* __restrict__ keys, int* __restrict__ sg) {
+    __shared__ int off_lds[kSegLds];
+    const int* off = stage_offsets(node_off, nseg, off_lds);
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
         const unsigned int v = vals[k];
         keys[k] = pkey[v];
-        sg[k] = seg_of[v];
+        sg[k] = graph_of(off, nseg, v);
     }
 }
 
+// One element per thread up to 2^24 elements (these kernels gather at random: more waves in
+// flight hide more of the latency), grid-stride beyond.
 int grid_for(int n) {
     int b = (n + 255) / 256;
-    return b < 1 ? 1 : (b > 4096 ? 4096 : b);
+    return b < 1 ? 1 : (b > 65536 ? 65536 : b);
 }
 
 }  // namespace
@@ -108,9 +140,11 @@ void launch_seg_keys(const Bufs& b, hipStream_t s) {
 }
 
 void launch_seg_graph_sort(Bufs& b, hipStream_t s) {
-    k_seg_gkeys<<<grid_for(b.n), 256, 0, s>>>(b.sort.vals[0], b.n, b.seg_of, b.sort.keys[0]);
+    // each block stages the graph offsets once: a grid of at most 2,048 blocks, grid-stride
+    const int gb = std::min(grid_for(b.n), 2048);
+    k_seg_gkeys<<<gb, 256, 0, s>>>(b.sort.vals[0], b.n, b.node_off, b.nseg, b.sort.keys[0]);
     radix_sort_pairs(b.sort, b.n, 0, (b.seg_bits + 7) & ~7, s);
-    k_seg_regather<<<grid_for(b.n), 256, 0, s>>>(b.sort.vals[0], b.n, b.seg_of, b.pkey, b.sort.keys[0], b.sg);
+    k_seg_regather<<<gb, 256, 0, s>>>(b.sort.vals[0], b.n, b.node_off, b.nseg, b.pkey, b.sort.keys[0], b.sg);
 }
 
 }  // namespace gfa2
