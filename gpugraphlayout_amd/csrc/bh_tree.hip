@@ -231,18 +231,41 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
                                                       const float* __restrict__ mass,
                                                       const float4* __restrict__ seg_root,
                                                       Mass3* __restrict__ bmass, float4* __restrict__ sbody) {
-    const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
+    // the tile's keys (with one on each side), graph ids and body ids, read coalesced into
+    // LDS; each thread then takes its kTreeItems consecutive positions from there
+    __shared__ unsigned long long skey[kTreeTile + 2];
+    __shared__ int ssg[kTreeTile + 2];
+    __shared__ unsigned int sval[kTreeTile];
+    const int base = blockIdx.x * kTreeTile;
+    for (int t = threadIdx.x; t < kTreeTile + 2; t += kTreeThreads) {
+        const int gi = base - 1 + t;
+        const bool in = gi >= 0 && gi < n;
+        skey[t] = in ? keys[gi] : 0ull;
+        if (sg) ssg[t] = in ? sg[gi] : -1;
+        if (t < kTreeTile) sval[t] = base + t < n ? vals[base + t] : 0u;
+    }
+    __syncthreads();
+    // seg_start and lcp_at over the staged tile (position i at index i - base + 1); the
+    // position before the tile is only asked for its lcp when no graph starts at the tile
+    auto seg_start_l = [&](int i) { return i == 0 || (sg != nullptr && ssg[i - base] != ssg[i - base + 1]); };
+    auto lcp_at_l = [&](int i) -> int {
+        if (i < 0) return -1;
+        const bool end = i >= n - 1 || (sg != nullptr && ssg[i - base + 1] != ssg[i - base + 2]);
+        if (end) return seg_start_l(i) ? 0 : -1;
+        return lcp_levels(skey[i - base + 1], skey[i - base + 2]);
+    };
+    const int i0 = base + threadIdx.x * kTreeItems;
     int sum = 0;
     Mass3 ms{0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < kTreeItems; ++k) {
         int i = i0 + k;
         if (i < n) {
-            const int prev = seg_start(sg, i) ? -1 : lcp_at(keys, sg, n, i - 1);
-            int l = lcp_at(keys, sg, n, i);
+            const int prev = seg_start_l(i) ? -1 : lcp_at_l(i - 1);
+            int l = lcp_at_l(i);
             lcp[i] = (signed char)l;
             sum += max(0, l - prev);
-            const unsigned int id = vals[i];
+            const unsigned int id = sval[i - base];
             const float2 p = pos[id];
             const float4 bd = make_float4(p.x, p.y, mass[id], 0.0f);
             sbody[i] = bd;
