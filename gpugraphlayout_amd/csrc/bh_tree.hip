@@ -353,15 +353,27 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
                                                          const float4* __restrict__ seg_root,
                                                          const Mass3* __restrict__ bmass, Mass3* __restrict__ pmass,
                                                          int* __restrict__ cell_pos, int max_cells) {
-    const int i0 = blockIdx.x * kTreeTile + threadIdx.x * kTreeItems;
+    // the tile's sorted bodies and graph ids (one before the tile), read coalesced into LDS
+    __shared__ float4 sbd[kTreeTile];
+    __shared__ int ssg[kTreeTile + 1];
+    const int base = blockIdx.x * kTreeTile;
+    for (int t = threadIdx.x; t < kTreeTile + 1; t += kTreeThreads) {
+        const int gi = base - 1 + t;
+        if (sg) ssg[t] = gi >= 0 && gi < n ? sg[gi] : -1;
+        if (t < kTreeTile) sbd[t] = base + t < n ? sbody[base + t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    __syncthreads();
+    const int i0 = base + threadIdx.x * kTreeItems;
     int s[kTreeItems];
     float4 bd[kTreeItems];
     int sum = 0;
 #pragma unroll
     for (int k = 0; k < kTreeItems; ++k) {
         int i = i0 + k;
-        s[k] = i < n ? max(0, (int)lcp[i] - lcp_prev(lcp, sg, i)) : 0;
-        bd[k] = i < n ? sbody[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // lcp_prev over the staged graph ids: no graph starts at i
+        const bool start = i == 0 || (sg != nullptr && ssg[i - base] != ssg[i - base + 1]);
+        s[k] = i < n ? max(0, (int)lcp[i] - (start ? -1 : (int)lcp[i - 1])) : 0;
+        bd[k] = sbd[i - base];
         sum += s[k];
     }
     int total;
@@ -372,7 +384,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
         Mass3 mine{0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < kTreeItems; ++k)
-            if (i0 + k < n) mine = m3_add(mine, body_m3(bd[k], sg ? sg[i0 + k] : 0, seg_root));
+            if (i0 + k < n) mine = m3_add(mine, body_m3(bd[k], sg ? ssg[i0 + k - base + 1] : 0, seg_root));
         Mass3 tot;
         Mass3 pre = m3_add(block_exclusive_scan3<kTreeThreads>(mine, &tot), bmass[blockIdx.x]);
 #pragma unroll
@@ -380,7 +392,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
             const int i = i0 + k;
             if (i < n) {
                 pmass[i] = pre;
-                pre = m3_add(pre, body_m3(bd[k], sg ? sg[i] : 0, seg_root));
+                pre = m3_add(pre, body_m3(bd[k], sg ? ssg[i - base + 1] : 0, seg_root));
                 if (i == n - 1) pmass[n] = pre;
             }
         }
