@@ -255,6 +255,16 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
         return lcp_levels(skey[i - base + 1], skey[i - base + 2]);
     };
     const int i0 = base + threadIdx.x * kTreeItems;
+    // all gathers of the thread's items first, outside the per-item branch, so they are in
+    // flight together (ids past n are 0, a valid node)
+    float2 pp[kTreeItems];
+    float mm[kTreeItems];
+#pragma unroll
+    for (int k = 0; k < kTreeItems; ++k) {
+        const unsigned int id = sval[i0 + k - base];
+        pp[k] = pos[id];
+        mm[k] = mass[id];
+    }
     int sum = 0;
     Mass3 ms{0.0, 0.0, 0.0};
 #pragma unroll
@@ -265,11 +275,9 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
             int l = lcp_at_l(i);
             lcp[i] = (signed char)l;
             sum += max(0, l - prev);
-            const unsigned int id = sval[i - base];
-            const float2 p = pos[id];
-            const float4 bd = make_float4(p.x, p.y, mass[id], 0.0f);
+            const float4 bd = make_float4(pp[k].x, pp[k].y, mm[k], 0.0f);
             sbody[i] = bd;
-            if (bmass) ms = m3_add(ms, body_m3(bd, sg ? sg[i] : 0, seg_root));
+            if (bmass) ms = m3_add(ms, body_m3(bd, sg ? ssg[i - base + 1] : 0, seg_root));
         }
     }
     int total;
