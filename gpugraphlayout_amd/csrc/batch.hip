@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_seg_keys(const float2* __restrict__ pos
             y += by ? r : -r;
         }
         keys[i] = key;
-        pkey[i] = key;
+        if (pkey) pkey[i] = key;      // kept for the regather after a global graph sort
         vals[i] = (unsigned)i;
     }
 }
@@ -136,7 +136,8 @@ void launch_seg_bbox(const Bufs& b, hipStream_t s) {
 }
 
 void launch_seg_keys(const Bufs& b, hipStream_t s) {
-    k_seg_keys<<<grid_for(b.n), 256, 0, s>>>(b.pos, b.n, b.seg_of, b.seg_root, b.sort.keys[0], b.sort.vals[0], b.pkey);
+    k_seg_keys<<<grid_for(b.n), 256, 0, s>>>(b.pos, b.n, b.seg_of, b.seg_root, b.sort.keys[0], b.sort.vals[0],
+                                             b.max_seg <= kSegSortMax ? nullptr : b.pkey);
 }
 
 void launch_seg_graph_sort(Bufs& b, hipStream_t s) {

@@ -175,8 +175,13 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
         launch_keys(b, s);
     }
     if (ev) hipEventRecord(ev[2], s);
-    radix_sort_pairs(b.sort, b.n, 0, 64, s);
-    if (b.nseg > 1) launch_seg_graph_sort(b, s);
+    if (b.nseg > 1 && b.max_seg <= kSegSortMax) {
+        // every graph fits one block's sort: by (graph, key) in one launch
+        segmented_sort_pairs(b.sort, b.node_off, b.seg_order, b.nseg, s);
+    } else {
+        radix_sort_pairs(b.sort, b.n, 0, 64, s);
+        if (b.nseg > 1) launch_seg_graph_sort(b, s);
+    }
     if (ev) hipEventRecord(ev[3], s);
     launch_tree(b, s);
     if (ev) hipEventRecord(ev[4], s);
@@ -490,6 +495,19 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
         GFA2_ALLOC(b.sg, n);
         GFA2_ALLOC(b.pkey, n);
         GFA2_TRY(hipMemcpyAsync(so, seg_of.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+        // a graph's sorted positions are its own node range, so sorted position -> graph is
+        // the node -> graph table (the global sort path rewrites it each iteration)
+        GFA2_TRY(hipMemcpyAsync(b.sg, seg_of.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+        // graphs by decreasing node count, for the per-graph sort
+        std::vector<int> order((size_t)nseg);
+        for (int g = 0; g < nseg; ++g) order[g] = g;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return node_off[x + 1] - node_off[x] > node_off[y + 1] - node_off[y];
+        });
+        b.max_seg = node_off[order[0] + 1] - node_off[order[0]];
+        GFA2_ALLOC(b.seg_order, nseg);
+        GFA2_TRY(hipMemcpyAsync(b.seg_order, order.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, h->stream));
+        GFA2_TRY(hipStreamSynchronize(h->stream));   // order goes out of scope
     }
     h->rep_buf = nullptr;
     if (h->capture_rep) {

@@ -160,6 +160,8 @@ struct Bufs {
     int2* groups = nullptr;           // walk groups {first sorted position, graph}, per graph in order
     int ngroups = 0;
     int* grp_first = nullptr;         // graph -> first group, nseg + 1
+    int* seg_order = nullptr;         // graphs by decreasing node count (per-graph sort)
+    int max_seg = 0;                  // largest graph's node count
     struct SegState* seg_st = nullptr;
     WalkPart* grp_part = nullptr;     // per group
     // Integer masses (all whole numbers, total < 2^24 * 62): a cell lighter than 2^24 has
@@ -195,6 +197,11 @@ constexpr int kTreeTile = kTreeThreads * kTreeItems;
 // Radix sort of (key, value) pairs on 64-bit keys, ascending and stable. Sorts
 // buf.keys[0]/vals[0] in place (via buf.keys[1]/vals[1]).
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s);
+// Sorts each segment [node_off[g], node_off[g+1]) of buf.keys[0]/vals[0] in place by the full
+// 64-bit key, stable, one block per segment (taken in `order`); for batches whose graphs all
+// have at most kSegSortMax nodes.
+void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s);
+constexpr int kSegSortMax = 32768;
 
 // Launch groups of one iteration, in order.
 void launch_attraction(const Bufs& b, const Params& p, hipStream_t s);

@@ -149,7 +149,7 @@ struct TileLds {
 // places. dglobal: this thread's digit's first destination for the tile. The pairs are ranked
 // stably (rank_rows), placed in LDS in digit order, then written out in tile order, so the
 // pairs of a digit leave as one run of consecutive addresses.
-__device__ __forceinline__ void scatter_tile(const unsigned long long (&key)[kSortItems],
+__device__ __forceinline__ unsigned int scatter_tile(const unsigned long long (&key)[kSortItems],
                                              const unsigned int (&val)[kSortItems], int tile, int n, int shift,
                                              unsigned int dglobal, TileLds& L,
                                              unsigned long long* __restrict__ kout, unsigned int* __restrict__ vout) {
@@ -194,6 +194,7 @@ __device__ __forceinline__ void scatter_tile(const unsigned long long (&key)[kSo
         kout[dst] = k;
         vout[dst] = L.sval[j];
     }
+    return tcount;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long long* __restrict__ kin,
@@ -220,6 +221,64 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long lo
     // digit starts: exclusive scan of the digit totals, plus this tile's offset within the digit
     const unsigned int dglobal = block_exscan256(rowtot[tid], L.sh[0]) + hist[(size_t)tid * tiles + blockIdx.x];
     scatter_tile(key, val, blockIdx.x, n, shift, dglobal, L, kout, vout);
+}
+
+// ---- one block per segment ----------------------------------------------------------------
+// Segment g is [node_off[g], node_off[g+1]) of both buffers; its pairs are sorted in place by
+// the full key, stable, by one block: the digit histograms of all passes first, then per pass
+// the segment's tiles in order, each ranked stably and written to its sorted place in the
+// other buffer (the running digit starts carry from tile to tile). Eight passes end in buffer
+// 0. Blocks take segments in `order` (largest first), so the longest ones start first.
+__global__ __launch_bounds__(kSortThreads) void k_seg_sort(unsigned long long* __restrict__ k0,
+                                                           unsigned int* __restrict__ v0,
+                                                           unsigned long long* __restrict__ k1,
+                                                           unsigned int* __restrict__ v1,
+                                                           const int* __restrict__ node_off,
+                                                           const int* __restrict__ order) {
+    __shared__ TileLds L;
+    __shared__ unsigned int hist[8][256];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int g = order[blockIdx.x];
+    const int a = node_off[g];
+    const int m = node_off[g + 1] - a;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) hist[p][tid] = 0u;
+    __syncthreads();
+    for (int e = tid; e < m; e += kSortThreads) {
+        const unsigned long long k = k0[a + e];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) atomicAdd(&hist[p][(unsigned)(k >> (8 * p)) & 255u], 1u);
+    }
+    __syncthreads();
+    const int tiles = (m + kSortTile - 1) / kSortTile;
+    for (int p = 0; p < 8; ++p) {
+        const unsigned long long* kin = ((p & 1) ? k1 : k0) + a;
+        const unsigned int* vin = ((p & 1) ? v1 : v0) + a;
+        unsigned long long* kout = ((p & 1) ? k0 : k1) + a;
+        unsigned int* vout = ((p & 1) ? v0 : v1) + a;
+        // this thread's digit: its first destination in the segment, advanced tile by tile
+        unsigned int dstart = block_exscan256(hist[p][tid], L.sh[0]);
+        for (int t = 0; t < tiles; ++t) {
+            const int wbase = t * kSortTile + wave * (kSortItems * 64);
+            unsigned long long key[kSortItems];
+            unsigned int val[kSortItems];
+#pragma unroll
+            for (int r = 0; r < kSortItems; ++r) {
+                const int e = wbase + r * 64 + lane;
+                key[r] = e < m ? kin[e] : 0ull;
+                val[r] = e < m ? vin[e] : 0u;
+            }
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) L.wcnt[w][tid] = 0;
+            // the counts are cleared (and the previous tile's LDS reads done) before ranking
+            __syncthreads();
+            dstart += scatter_tile(key, val, t, m, 8 * p, dstart, L, kout, vout);
+        }
+        // this pass's writes are read by the next one (one block: workgroup-scope visibility)
+        __syncthreads();
+    }
 }
 
 // ---- onesweep -------------------------------------------------------------------------------
@@ -320,6 +379,11 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
 }
 
 }  // namespace
+
+void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s) {
+    if (nseg > 0)
+        k_seg_sort<<<nseg, kSortThreads, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], node_off, order);
+}
 
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s) {
     if (n <= 1) return;

@@ -105,7 +105,8 @@ int gfa2_set_graph_coo(gfa2_handle h, int32_t n, int64_t m, const int32_t* src, 
  * holds one graph per process): graph g owns nodes [node_off[g], node_off[g+1]) of a
  * block-diagonal symmetric CSR (no edge may leave its graph). Each graph gets its own root
  * box, tree, walk groups and speed state, so it follows the trajectory it would follow
- * alone. Positions are the concatenation. */
+ * alone: the same tree, masses and counts, with cell centroids (FP64 prefix sums over the
+ * batch, for integer masses) equal to the last float bit. Positions are the concatenation. */
 int gfa2_set_graph_batch(gfa2_handle h, int32_t ngraphs, const int32_t* node_off, const int64_t* rowptr,
                          const int32_t* col, const float* mass);
 /* Per-graph speed, efficiency, swing, traction and interaction count of the last

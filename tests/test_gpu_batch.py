@@ -1,4 +1,5 @@
-"""A batch of graphs laid out together equals, bit for bit, each graph laid out alone."""
+"""A batch of graphs laid out together against each graph laid out alone: bit for bit on small
+graphs, to float rounding where a cell centroid's FP64 prefix sum rounds differently in the batch."""
 import numpy as np
 import pytest
 
@@ -7,9 +8,13 @@ from gpugraphlayout_amd import BatchEngine, Engine, graphs
 pytestmark = pytest.mark.gpu
 
 
-def batch_case(seed):
+def batch_case(seed, big=False):
     out = []
-    for k, (n, m) in enumerate([(300, 3), (50, 2), (1000, 5), (2, 1), (120, 4), (7, 1)]):
+    sizes = [(300, 3), (50, 2), (1000, 5), (2, 1), (120, 4), (7, 1)]
+    if big:
+        # one graph over 32,768 nodes: the batch sorts globally, then by graph
+        sizes.insert(2, (40000, 2))
+    for k, (n, m) in enumerate(sizes):
         s, t = graphs.barabasi_albert(n, m, seed + k)
         g = graphs.edges_to_graph(s, t)
         out.append(g)
@@ -39,6 +44,40 @@ def test_batch_equals_single_runs(group):
         s1 = e.stats()
         assert st["speed"][k] == np.float32(s1["global_speed"])
         assert st["swing"][k] == s1["swing"]
+        e.close()
+    be.close()
+
+
+def test_batch_with_a_graph_over_the_per_graph_sort_limit():
+    """A 40,000-node graph sends the batch through the global sort and then the graph sort.
+    Tree shape, masses and counts match the single runs exactly; a cell centroid comes from
+    FP64 prefix sums over the batch's sorted bodies, whose rounding depends on the graphs
+    before it, so it can land on the neighbouring float: the runs agree to float rounding."""
+    print("case batch with a 40,000-node graph", flush=True)
+    gs = batch_case(3, big=True)
+    pos = [graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 40 + k) for k, g in enumerate(gs)]
+    params = dict(scale=2.0, gravity=1.0, strong_gravity=1, group_size=32)
+    be = BatchEngine([(g.n, g.src, g.dst) for g in gs], **params)
+    be.set_positions(np.concatenate(pos))
+    singles = []
+    for k, g in enumerate(gs):
+        e = Engine(g.n, g.src, g.dst, **params)
+        e.set_positions(pos[k])
+        singles.append(e)
+    for steps, tol in ((1, 1e-6), (2, 1e-5)):
+        be.step(1)
+        be.sync()
+        got = be.split(be.positions())
+        st = be.batch_stats()
+        for k, e in enumerate(singles):
+            e.step(1)
+            e.sync()
+            want = e.positions()
+            rel = np.linalg.norm(got[k] - want) / np.linalg.norm(want)
+            assert rel <= tol, (k, steps, rel)
+            inter = e.stats()["interactions"]
+            assert abs(int(st["interactions"][k]) - inter) <= 1e-6 * inter + 1, (k, steps)
+    for e in singles:
         e.close()
     be.close()
 
