@@ -236,22 +236,28 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_sort(unsigned long long* _
                                                            const int* __restrict__ node_off,
                                                            const int* __restrict__ order) {
     __shared__ TileLds L;
-    __shared__ unsigned int hist[8][256];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int g = order[blockIdx.x];
     const int a = node_off[g];
     const int m = node_off[g + 1] - a;
+    // the 8 digit histograms, counted in the (not yet used) tile staging area, then kept per
+    // thread for its own digit: hc[0] is the current pass's count, shifted down after each pass
+    unsigned int* hist = L.sval;                 // 2,048 words: 8 x 256
 #pragma unroll
-    for (int p = 0; p < 8; ++p) hist[p][tid] = 0u;
+    for (int p = 0; p < 8; ++p) hist[p * 256 + tid] = 0u;
     __syncthreads();
     for (int e = tid; e < m; e += kSortThreads) {
         const unsigned long long k = k0[a + e];
 #pragma unroll
-        for (int p = 0; p < 8; ++p) atomicAdd(&hist[p][(unsigned)(k >> (8 * p)) & 255u], 1u);
+        for (int p = 0; p < 8; ++p) atomicAdd(&hist[p * 256 + ((unsigned)(k >> (8 * p)) & 255u)], 1u);
     }
     __syncthreads();
+    unsigned int hc[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) hc[p] = hist[p * 256 + tid];
+    __syncthreads();                             // the staging area is the tiles' again
     const int tiles = (m + kSortTile - 1) / kSortTile;
     for (int p = 0; p < 8; ++p) {
         const unsigned long long* kin = ((p & 1) ? k1 : k0) + a;
@@ -259,7 +265,9 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_sort(unsigned long long* _
         unsigned long long* kout = ((p & 1) ? k0 : k1) + a;
         unsigned int* vout = ((p & 1) ? v0 : v1) + a;
         // this thread's digit: its first destination in the segment, advanced tile by tile
-        unsigned int dstart = block_exscan256(hist[p][tid], L.sh[0]);
+        unsigned int dstart = block_exscan256(hc[0], L.sh[0]);
+#pragma unroll
+        for (int q = 0; q < 7; ++q) hc[q] = hc[q + 1];
         for (int t = 0; t < tiles; ++t) {
             const int wbase = t * kSortTile + wave * (kSortItems * 64);
             unsigned long long key[kSortItems];
