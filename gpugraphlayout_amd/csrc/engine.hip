@@ -766,14 +766,19 @@ int gfa2_get_tree(gfa2_handle h, int32_t cap, int32_t* ncells, int32_t* level, i
     }
     if (perm)
         GFA2_TRY(hipMemcpyAsync(perm, h->b.sort.vals[0], sizeof(int) * h->b.n, hipMemcpyDeviceToHost, h->stream));
+    // each graph's root record (a cell stores level - 1, and a root level 0 as 0 too)
+    std::vector<int> root((size_t)h->b.nseg + 1, 0);
+    GFA2_TRY(hipMemcpyAsync(root.data(), h->b.rec_begin, sizeof(int) * root.size(), hipMemcpyDeviceToHost, h->stream));
     GFA2_TRY(hipStreamSynchronize(h->stream));
     int c = 0;
+    size_t g = 0;
     for (int r = 0; r < nrec; ++r) {
         int info;
         memcpy(&info, &rec[r].w, sizeof(int));
         if (info == -1) continue;
+        while (g + 1 < root.size() && root[g] < r) ++g;
         if (c < cap) {
-            if (level) level[c] = r == 0 ? 0 : (info & 31) + 1;
+            if (level) level[c] = (g < root.size() && root[g] == r) ? 0 : (info & 31) + 1;
             if (count) count[c] = rcount[r];
             if (mass) mass[c] = rec[r].z;
             if (xy) { xy[2 * c] = rec[r].x; xy[2 * c + 1] = rec[r].y; }

@@ -99,3 +99,36 @@ def test_batch_with_single_node_graph():
     e.step(10)
     e.sync()
     np.testing.assert_array_equal(p[1:].view(np.uint32), e.positions().view(np.uint32))
+
+
+def test_batch_tree_matches_single_trees_across_tile_boundaries():
+    """Graphs of 2,047..12,000 nodes (one to six sort tiles each, sorted one block per graph):
+    the in-order permutation and every cell's level, body count and mass equal the graph's own
+    tree; centroids agree to float rounding."""
+    print("case batch tree across tile boundaries", flush=True)
+    sizes = [(2047, 2), (40, 2), (2048, 3), (2049, 2), (3, 1), (5000, 4), (12000, 2)]
+    gs = [graphs.edges_to_graph(*graphs.barabasi_albert(n, m, 11 + k)) for k, (n, m) in enumerate(sizes)]
+    pos = [graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 70 + k) for k, g in enumerate(gs)]
+    be = BatchEngine([(g.n, g.src, g.dst) for g in gs], scale=2.0)
+    be.set_positions(np.concatenate(pos))
+    be.step(1)
+    be.sync()
+    T = be.tree()
+    off = np.concatenate([[0], np.cumsum([g.n for g in gs])])
+    c0 = 0
+    for k, g in enumerate(gs):
+        e = Engine(g.n, g.src, g.dst, scale=2.0)
+        e.set_positions(pos[k])
+        e.step(1)
+        e.sync()
+        S = e.tree()
+        np.testing.assert_array_equal(T["perm"][off[k]:off[k + 1]] - off[k], S["perm"])
+        c1 = c0 + len(S["level"])
+        for key in ("level", "count", "mass"):
+            np.testing.assert_array_equal(T[key][c0:c1], S[key], err_msg=f"graph {k} {key}")
+        np.testing.assert_allclose(T["xy"][c0:c1], S["xy"], rtol=1e-6, atol=1e-6 * np.abs(S["xy"]).max())
+        assert S["level"][0] == 0 and T["level"][c0] == 0
+        c0 = c1
+        e.close()
+    assert c0 == len(T["level"])
+    be.close()
