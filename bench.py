@@ -343,7 +343,7 @@ def main():
     ap.add_argument("--group", type=int, default=32)
     ap.add_argument("--cpu-iters", type=int, default=0, help="CPU-baseline iterations (0: the workload's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--autotune", type=int, default=10,
+    ap.add_argument("--autotune", type=int, default=30,
                     help="iterations per candidate launch shape before timing (0: default shapes)")
     ap.add_argument("--walk-block", type=int, default=0, help="fixed walk block size (skips the autotuner)")
     ap.add_argument("--blocks-per-cu", type=int, default=8, help="with --walk-block: grid-stride blocks per CU")
