@@ -274,6 +274,18 @@ def run_batch(args):
     log(f"warm-up done ({args.warmup} it), timing {steps} it")
     dt, groups, s0, s1, launch, tune_s = timed_run(eng, args, steps)
     value = steps / dt
+    replay = None
+    if args.graph_replay > 0:
+        # the whole batch's iteration captured once as a hipGraph, then replayed
+        eng.set_graph_capture(True)
+        eng.step(1)
+        eng.sync()
+        t1 = time.perf_counter()
+        eng.step(args.graph_replay)
+        eng.sync()
+        replay = args.graph_replay / (time.perf_counter() - t1)
+        eng.set_graph_capture(False)
+        log(f"hipGraph replay: {replay:.1f} it/s")
     eng.close()
 
     rng = np.random.default_rng(7)
@@ -328,6 +340,7 @@ def run_batch(args):
         "sequential_sample_graphs": int(len(sample)),
         "batched_s": dt,
         "batched_speedup_vs_sequential": seq_total / dt,
+        "graph_replay_iterations_per_s": replay,
         "autotune_s": tune_s,
         "device": dev,
     })

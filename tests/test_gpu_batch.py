@@ -132,3 +132,22 @@ def test_batch_tree_matches_single_trees_across_tile_boundaries():
         e.close()
     assert c0 == len(T["level"])
     be.close()
+
+
+def test_batch_graph_replay_matches_eager():
+    """The batch's iteration captured as one hipGraph and replayed gives the eager result."""
+    print("case batch hipGraph replay", flush=True)
+    gs = batch_case(5)
+    pos = np.concatenate([graphs.uniform_positions(g.n, 2 * np.sqrt(g.n), 90 + k) for k, g in enumerate(gs)])
+    out = []
+    for capture in (False, True):
+        be = BatchEngine([(g.n, g.src, g.dst) for g in gs], scale=2.0)
+        be.set_graph_capture(capture)
+        be.set_positions(pos)
+        for _ in range(3):
+            be.step(4)
+            be.sync()
+        out.append((be.positions(), be.batch_stats()))
+        be.close()
+    np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    np.testing.assert_array_equal(out[0][1]["speed"], out[1][1]["speed"])
