@@ -389,7 +389,22 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restr
     const int g0 = grp_first[g], g1 = grp_first[g + 1];
     double s = 0.0, t = 0.0;
     unsigned long long it = 0;
-    for (int k = g0 + tid; k < g1; k += kSpeedThreads) {
+    // thread-strided sums in group order; 8 partials are loaded before they are added (same
+    // order, so the same bits), keeping 8 loads in flight per thread
+    constexpr int kAhead = 8;
+    int k = g0 + tid;
+    for (; k + (kAhead - 1) * kSpeedThreads < g1; k += kAhead * kSpeedThreads) {
+        WalkPart p[kAhead];
+#pragma unroll
+        for (int j = 0; j < kAhead; ++j) p[j] = part[k + j * kSpeedThreads];
+#pragma unroll
+        for (int j = 0; j < kAhead; ++j) {
+            s += (double)p[j].swing;
+            t += (double)p[j].traction;
+            it += p[j].interactions;
+        }
+    }
+    for (; k < g1; k += kSpeedThreads) {
         const WalkPart p = part[k];
         s += (double)p.swing;
         t += (double)p.traction;
