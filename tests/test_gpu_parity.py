@@ -297,3 +297,27 @@ def test_forces_on_awkward_shapes(oracle, kind):
     fga = oracle.attraction_coo(pos, g.src, g.dst, oracle.gravity(pos, g.mass(), False, 1.0))
     assert rel_l2(out["fga"], fga) <= 1e-5
     assert np.all(np.isfinite(out["pos"]))
+
+
+@pytest.mark.parametrize("kind", ["rmat", "ba"])
+def test_attraction_depends_on_positions_only(kind):
+    """Gravity + attraction depend on the positions only, not on the engine's history: after 41
+    iterations they equal, bit for bit, a fresh engine's on the same positions."""
+    print(f"case attraction on positions only {kind}", flush=True)
+    g, pos = make_case(kind, 51)
+    a = Engine(g.n, g.src, g.dst, scale=10.0, gravity=1.0)
+    a.set_positions(pos)
+    a.step(41)
+    a.sync()
+    p41 = a.positions()
+    a.step(1)
+    a.sync()
+    fa = a.forces(GFA2_FORCE_ATTRACTION_GRAVITY)
+    b = Engine(g.n, g.src, g.dst, scale=10.0, gravity=1.0)
+    b.set_positions(p41)
+    b.step(1)
+    b.sync()
+    fb = b.forces(GFA2_FORCE_ATTRACTION_GRAVITY)
+    np.testing.assert_array_equal(fa.view(np.uint32), fb.view(np.uint32))
+    a.close()
+    b.close()
