@@ -191,7 +191,7 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kTreeThreads = 256;
-constexpr int kTreeItems = 8;
+constexpr int kTreeItems = 2;
 constexpr int kTreeTile = kTreeThreads * kTreeItems;
 
 // Radix sort of (key, value) pairs on 64-bit keys, ascending and stable. Sorts
