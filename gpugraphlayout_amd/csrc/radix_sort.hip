@@ -242,9 +242,12 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_sort(unsigned long long* _
     const int g = order[blockIdx.x];
     const int a = node_off[g];
     const int m = node_off[g + 1] - a;
-    // the 8 digit histograms, counted in the (not yet used) tile staging area, then kept per
-    // thread for its own digit: hc[0] is the current pass's count, shifted down after each pass
-    unsigned int* hist = L.sval;                 // 2,048 words: 8 x 256
+    // the 8 digit histograms (8 x 256 words), counted in the not yet used tile staging area when
+    // it holds them, then kept per thread for its own digit: hc[0] is the current pass's count,
+    // shifted down after each pass
+    constexpr bool kAlias = kSortTile >= 8 * 256;
+    __shared__ unsigned int hist_own[kAlias ? 1 : 8 * 256];
+    unsigned int* hist = kAlias ? L.sval : hist_own;
 #pragma unroll
     for (int p = 0; p < 8; ++p) hist[p * 256 + tid] = 0u;
     __syncthreads();
