@@ -492,15 +492,23 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
             rec[rc].w = __uint_as_float(info);
         }
         rcount[rc] = b - i + 1;
-        cell_rec[ord] = rc;
-        cell_lvl[ord] = (unsigned char)L;
+        if (!pmass) {
+            // the per-level cell lists of the summary launches (not needed for integer masses)
+            cell_rec[ord] = rc;
+            cell_lvl[ord] = (unsigned char)L;
+            atomicAdd(&hist[L], 1);
+        }
         if (L == 0 && rec_begin) rec_begin[gi] = rc;
-        atomicAdd(&hist[L], 1);
         atomicMax(&lmax, L);
     }
     __syncthreads();
-    if (threadIdx.x <= kMaxLevels && hist[threadIdx.x] != 0) atomicAdd(&st->level_count[threadIdx.x], hist[threadIdx.x]);
-    if (threadIdx.x == 0 && lmax > 0) atomicMax(&st->max_level, lmax);
+    if (!pmass && threadIdx.x <= kMaxLevels && hist[threadIdx.x] != 0)
+        atomicAdd(&st->level_count[threadIdx.x], hist[threadIdx.x]);
+    // every block would hit this one word: only blocks above the value seen so far update it
+    // (the maximum only grows, so skipping on a stale value is still correct)
+    if (threadIdx.x == 0 && lmax > 0 &&
+        lmax > __hip_atomic_load(&st->max_level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&st->max_level, lmax);
 }
 
 // Bucket cells by level (order inside a level is irrelevant).
