@@ -804,7 +804,16 @@ int gfa2_get_stats(gfa2_handle h, gfa2_stats* s) {
     s->swing = st.swing;
     s->traction = st.traction;
     s->interactions = (int64_t)st.interactions;
-    s->interactions_total = (int64_t)st.interactions_total;
+    {
+        // the running totals are kept per graph
+        std::vector<SegState> seg((size_t)h->b.nseg);
+        GFA2_TRY(hipMemcpyAsync(seg.data(), h->b.seg_st, sizeof(SegState) * seg.size(), hipMemcpyDeviceToHost,
+                                h->stream));
+        GFA2_TRY(hipStreamSynchronize(h->stream));
+        unsigned long long total = 0;
+        for (const SegState& q : seg) total += q.interactions_total;
+        s->interactions_total = (int64_t)total;
+    }
     s->cells = st.ncells;
     s->max_level = st.max_level;
     s->root_x = st.rootx;

@@ -42,7 +42,7 @@ struct DevState {
     float global_speed, speed_efficiency, jitter_tolerance, k_s_max;
     double swing, traction;
     unsigned long long interactions;
-    unsigned long long interactions_total;   // summed over all iterations since reset
+    unsigned long long interactions_total;   // unused: kept per graph in SegState
     int ncells;
     int nrecords;
     int max_level;      // deepest cell level
@@ -180,6 +180,7 @@ struct SegState {
     double swing, traction;
     unsigned long long interactions;
     int iteration, pad;
+    unsigned long long interactions_total;   // summed over all iterations since reset
 };
 
 // Batched-mode launch groups (batch.hip).

@@ -458,7 +458,8 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restr
             st->speed_efficiency = se;
         }
     }
-    if (tid == 0) atomicAdd(&st->interactions_total, ri[0]);
+    // per graph (one block each): no word shared by the blocks of a batch
+    if (tid == 0) seg[g].interactions_total += ri[0];
 }
 
 // ---------------------------------------------------------------------------------------
