@@ -299,22 +299,42 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int per = (nb + 1023) / 1024;
     const int b0 = tid * per;
+    // each thread's run of `per` tiles is read kAhead entries at a time with independent loads,
+    // then taken in order (the same additions, so the same bits)
+    constexpr int kAhead = 8;
+    auto valid = [&](int k0, int j) { return k0 + j < per && b0 + k0 + j < nb; };
     if (bmass) {
         Mass3 loc{0.0, 0.0, 0.0};
-        for (int k = 0; k < per; ++k)
-            if (b0 + k < nb) loc = m3_add(loc, bmass[b0 + k]);
+        for (int k0 = 0; k0 < per; k0 += kAhead) {
+            Mass3 c[kAhead];
+#pragma unroll
+            for (int j = 0; j < kAhead; ++j) c[j] = valid(k0, j) ? bmass[b0 + k0 + j] : Mass3{0.0, 0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < kAhead; ++j)
+                if (valid(k0, j)) loc = m3_add(loc, c[j]);
+        }
         Mass3 tot;
         Mass3 run = block_exclusive_scan3<1024>(loc, &tot);
-        for (int k = 0; k < per; ++k)
-            if (b0 + k < nb) {
-                const Mass3 c = bmass[b0 + k];
-                bmass[b0 + k] = run;
-                run = m3_add(run, c);
-            }
+        for (int k0 = 0; k0 < per; k0 += kAhead) {
+            Mass3 c[kAhead];
+#pragma unroll
+            for (int j = 0; j < kAhead; ++j) c[j] = valid(k0, j) ? bmass[b0 + k0 + j] : Mass3{0.0, 0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < kAhead; ++j)
+                if (valid(k0, j)) {
+                    bmass[b0 + k0 + j] = run;
+                    run = m3_add(run, c[j]);
+                }
+        }
     }
     int local = 0;
-    for (int k = 0; k < per; ++k)
-        if (b0 + k < nb) local += blocksum[b0 + k];
+    for (int k0 = 0; k0 < per; k0 += kAhead) {
+        int c[kAhead];
+#pragma unroll
+        for (int j = 0; j < kAhead; ++j) c[j] = valid(k0, j) ? blocksum[b0 + k0 + j] : 0;
+#pragma unroll
+        for (int j = 0; j < kAhead; ++j) local += c[j];
+    }
     int inc = local;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -329,12 +349,16 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum
         all += wsum[w];
     }
     int run = before + inc - local;
-    for (int k = 0; k < per; ++k) {
-        if (b0 + k < nb) {
-            int c = blocksum[b0 + k];
-            blocksum[b0 + k] = run;
-            run += c;
-        }
+    for (int k0 = 0; k0 < per; k0 += kAhead) {
+        int c[kAhead];
+#pragma unroll
+        for (int j = 0; j < kAhead; ++j) c[j] = valid(k0, j) ? blocksum[b0 + k0 + j] : 0;
+#pragma unroll
+        for (int j = 0; j < kAhead; ++j)
+            if (valid(k0, j)) {
+                blocksum[b0 + k0 + j] = run;
+                run += c[j];
+            }
     }
     if (tid == 0) {
         st->ncells = all;
