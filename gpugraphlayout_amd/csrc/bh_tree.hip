@@ -15,9 +15,13 @@
 //     the prefixes of length <= lcp_i; the cells starting at position i have levels
 //     lcp_{i-1}+1 .. lcp_i. Their pre-order record index follows from an exclusive scan
 //     of those counts, and each cell's extent from a galloping search over the keys;
-//  5. summaries bottom-up, one launch per level, children in quadrant order, mass
-//     accumulated from 0.0f and centroid (sum m*p)*(1/mass), as the reference's
-//     SummarizationKernel (src/RPBHKernels.cu:357-506). Counts are extents.
+//  5. summaries. Counts are extents. For whole-number masses (the default deg+1) a cell
+//     lighter than 2^24 has the same float mass in every summation order, so its mass and
+//     centroid come from exact integer prefix sums over the sorted bodies (fixed-point
+//     moments, see Moment); heavier cells are summed in quadrant order. Other masses:
+//     bottom-up, one launch per level, children in quadrant order, mass accumulated from
+//     0.0f and centroid (sum m*p)*(1/mass), as the reference's SummarizationKernel
+//     (src/RPBHKernels.cu:357-506).
 // Nothing waits on another workgroup; every dependency crosses a launch boundary.
 #include "engine_internal.h"
 
@@ -176,46 +180,89 @@ __device__ int block_exclusive_scan(int v, int* total) {
     return before + inc - v;
 }
 
-__device__ __forceinline__ Mass3 m3_add(Mass3 a, Mass3 b) { return Mass3{a.m + b.m, a.x + b.x, a.y + b.y}; }
+// Exact moments (integer-mass path). Sums are taken modulo 2^128 on two 64-bit limbs.
+using u128 = unsigned __int128;
 
-// Block-wide exclusive scan of one Mass3 per thread (NT threads); *total gets the sum.
+__device__ __forceinline__ u128 mo_x(const Moment& a) { return ((u128)a.xh << 64) | a.xl; }
+__device__ __forceinline__ u128 mo_y(const Moment& a) { return ((u128)a.yh << 64) | a.yl; }
+
+__device__ __forceinline__ Moment mo_make(u128 x, u128 y, unsigned long long m) {
+    return Moment{(unsigned long long)x, (unsigned long long)(x >> 64), (unsigned long long)y,
+                  (unsigned long long)(y >> 64), m};
+}
+
+__device__ __forceinline__ Moment mo_add(const Moment& a, const Moment& b) {
+    return mo_make(mo_x(a) + mo_x(b), mo_y(a) + mo_y(b), a.m + b.m);
+}
+
+__device__ __forceinline__ Moment mo_sub(const Moment& a, const Moment& b) {
+    return mo_make(mo_x(a) - mo_x(b), mo_y(a) - mo_y(b), a.m - b.m);
+}
+
+__device__ __forceinline__ Moment mo_shfl_up(const Moment& v, int o) {
+    return Moment{__shfl_up(v.xl, o, 64), __shfl_up(v.xh, o, 64), __shfl_up(v.yl, o, 64), __shfl_up(v.yh, o, 64),
+                  __shfl_up(v.m, o, 64)};
+}
+
+// Fixed-point exponent of a graph whose root radius is R < 2^e: a coordinate relative to the
+// root centre lies within a few R (the centre and radius are rounded floats), so scaled by
+// 2^(58-e) it stays below 2^61 in magnitude. R = 0 (one point) gives coordinates 0.
+__device__ __forceinline__ int moment_shift(float radius) {
+    int e = 0;
+    (void)frexpf(radius, &e);
+    return 58 - e;
+}
+
+__device__ __forceinline__ long long fixed_coord(float p, float c, int s) {
+    double t = ldexp((double)p - (double)c, s);
+    t = fmin(fmax(t, -0x1p62), 0x1p62);
+    return (long long)rint(t);
+}
+
+// The signed 128-bit value hi*2^64 + lo as a double (magnitude converted, so small negative
+// values keep their precision).
+__device__ __forceinline__ double i128_to_double(unsigned long long lo, unsigned long long hi) {
+    u128 v = ((u128)hi << 64) | lo;
+    const bool neg = (long long)hi < 0;
+    if (neg) v = (u128)0 - v;
+    const unsigned long long vh = (unsigned long long)(v >> 64), vl = (unsigned long long)v;
+    const double mag = vh == 0ull ? (double)vl : (double)vh * 0x1p64 + (double)vl;
+    return neg ? -mag : mag;
+}
+
+// Block-wide exclusive scan of one Moment per thread (NT threads); *total gets the sum.
 template <int NT>
-__device__ Mass3 block_exclusive_scan3(Mass3 v, Mass3* total) {
-    __shared__ double ws[3][NT / 64];
+__device__ Moment block_exclusive_scan_mo(const Moment& v, Moment* total) {
+    __shared__ Moment ws[NT / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double a = v.m, b = v.x, c = v.y;
+    Moment inc = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const double ta = __shfl_up(a, o, 64), tb = __shfl_up(b, o, 64), tc = __shfl_up(c, o, 64);
-        if (lane >= o) {
-            a += ta;
-            b += tb;
-            c += tc;
-        }
+        const Moment t = mo_shfl_up(inc, o);
+        if (lane >= o) inc = mo_add(inc, t);
     }
-    if (lane == 63) {
-        ws[0][wave] = a;
-        ws[1][wave] = b;
-        ws[2][wave] = c;
-    }
+    if (lane == 63) ws[wave] = inc;
     __syncthreads();
-    Mass3 before{0.0, 0.0, 0.0}, all{0.0, 0.0, 0.0};
+    Moment before{0ull, 0ull, 0ull, 0ull, 0ull}, all{0ull, 0ull, 0ull, 0ull, 0ull};
     for (int w = 0; w < NT / 64; ++w) {
-        const Mass3 s{ws[0][w], ws[1][w], ws[2][w]};
-        if (w < wave) before = m3_add(before, s);
-        all = m3_add(all, s);
+        const Moment s = ws[w];
+        if (w < wave) before = mo_add(before, s);
+        all = mo_add(all, s);
     }
     __syncthreads();
     *total = all;
-    return Mass3{before.m + a - v.m, before.x + b - v.x, before.y + c - v.y};
+    return mo_sub(mo_add(before, inc), v);
 }
 
-// Mass and mass-weighted position (relative to the graph's root centre) of a sorted body
-// {x, y, mass} of graph g.
-__device__ __forceinline__ Mass3 body_m3(const float4 bd, int g, const float4* __restrict__ seg_root) {
+// Mass and mass-weighted fixed-point position (relative to the graph's root centre) of a
+// sorted body {x, y, mass} of graph g; the mass is a whole number below 2^24.
+__device__ __forceinline__ Moment body_moment(const float4 bd, int g, const float4* __restrict__ seg_root) {
     const float4 c = seg_root[g];
-    const double m = bd.z;
-    return Mass3{m, m * ((double)bd.x - c.x), m * ((double)bd.y - c.y)};
+    const int s = moment_shift(c.z);
+    const unsigned long long m = (unsigned long long)bd.z;
+    const u128 x = (u128)(__int128)fixed_coord(bd.x, c.x, s) * (u128)m;
+    const u128 y = (u128)(__int128)fixed_coord(bd.y, c.y, s) * (u128)m;
+    return mo_make(x, y, m);
 }
 
 // lcp per sorted position and the number of cells opened there; block sums (of the
@@ -230,7 +277,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
                                                       const float2* __restrict__ pos,
                                                       const float* __restrict__ mass,
                                                       const float4* __restrict__ seg_root,
-                                                      Mass3* __restrict__ bmass, float4* __restrict__ sbody) {
+                                                      Moment* __restrict__ bmass, float4* __restrict__ sbody) {
     // the tile's keys (with one on each side), graph ids and body ids, read coalesced into
     // LDS; each thread then takes its kTreeItems consecutive positions from there
     __shared__ unsigned long long skey[kTreeTile + 2];
@@ -266,7 +313,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
         mm[k] = mass[id];
     }
     int sum = 0;
-    Mass3 ms{0.0, 0.0, 0.0};
+    Moment ms{0ull, 0ull, 0ull, 0ull, 0ull};
 #pragma unroll
     for (int k = 0; k < kTreeItems; ++k) {
         int i = i0 + k;
@@ -277,15 +324,15 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
             sum += max(0, l - prev);
             const float4 bd = make_float4(pp[k].x, pp[k].y, mm[k], 0.0f);
             sbody[i] = bd;
-            if (bmass) ms = m3_add(ms, body_m3(bd, sg ? ssg[i - base + 1] : 0, seg_root));
+            if (bmass) ms = mo_add(ms, body_moment(bd, sg ? ssg[i - base + 1] : 0, seg_root));
         }
     }
     int total;
     block_exclusive_scan(sum, &total);
     if (threadIdx.x == 0) blocksum[blockIdx.x] = total;
     if (bmass) {
-        Mass3 mt;
-        block_exclusive_scan3<kTreeThreads>(ms, &mt);
+        Moment mt;
+        block_exclusive_scan_mo<kTreeThreads>(ms, &mt);
         if (threadIdx.x == 0) bmass[blockIdx.x] = mt;
     }
 }
@@ -294,36 +341,36 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
 // per-iteration counters reset.
 __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum, int nb, int n,
                                                       int cap_records, int* __restrict__ rec_end, DevState* st,
-                                                      Mass3* __restrict__ bmass) {
+                                                      Moment* __restrict__ bmass) {
     __shared__ int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int per = (nb + 1023) / 1024;
     const int b0 = tid * per;
     // each thread's run of `per` tiles is read kAhead entries at a time with independent loads,
-    // then taken in order (the same additions, so the same bits)
+    // then taken in order
     constexpr int kAhead = 8;
     auto valid = [&](int k0, int j) { return k0 + j < per && b0 + k0 + j < nb; };
     if (bmass) {
-        Mass3 loc{0.0, 0.0, 0.0};
+        Moment loc{0ull, 0ull, 0ull, 0ull, 0ull};
         for (int k0 = 0; k0 < per; k0 += kAhead) {
-            Mass3 c[kAhead];
+            Moment c[kAhead];
 #pragma unroll
-            for (int j = 0; j < kAhead; ++j) c[j] = valid(k0, j) ? bmass[b0 + k0 + j] : Mass3{0.0, 0.0, 0.0};
+            for (int j = 0; j < kAhead; ++j) c[j] = valid(k0, j) ? bmass[b0 + k0 + j] : Moment{0ull, 0ull, 0ull, 0ull, 0ull};
 #pragma unroll
             for (int j = 0; j < kAhead; ++j)
-                if (valid(k0, j)) loc = m3_add(loc, c[j]);
+                if (valid(k0, j)) loc = mo_add(loc, c[j]);
         }
-        Mass3 tot;
-        Mass3 run = block_exclusive_scan3<1024>(loc, &tot);
+        Moment tot;
+        Moment run = block_exclusive_scan_mo<1024>(loc, &tot);
         for (int k0 = 0; k0 < per; k0 += kAhead) {
-            Mass3 c[kAhead];
+            Moment c[kAhead];
 #pragma unroll
-            for (int j = 0; j < kAhead; ++j) c[j] = valid(k0, j) ? bmass[b0 + k0 + j] : Mass3{0.0, 0.0, 0.0};
+            for (int j = 0; j < kAhead; ++j) c[j] = valid(k0, j) ? bmass[b0 + k0 + j] : Moment{0ull, 0ull, 0ull, 0ull, 0ull};
 #pragma unroll
             for (int j = 0; j < kAhead; ++j)
                 if (valid(k0, j)) {
                     bmass[b0 + k0 + j] = run;
-                    run = m3_add(run, c[j]);
+                    run = mo_add(run, c[j]);
                 }
         }
     }
@@ -383,7 +430,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
                                                          float4* __restrict__ rec, int* __restrict__ brec,
                                                          const DevState* __restrict__ st,
                                                          const float4* __restrict__ seg_root,
-                                                         const Mass3* __restrict__ bmass, Mass3* __restrict__ pmass,
+                                                         const Moment* __restrict__ bmass, Moment* __restrict__ pmass,
                                                          int* __restrict__ cell_pos, int max_cells) {
     // the tile's sorted bodies and graph ids (one before the tile), read coalesced into LDS
     __shared__ float4 sbd[kTreeTile];
@@ -413,18 +460,18 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
     const bool ok = st->err == 0;
     if (pmass) {
         // exclusive prefix of body mass sums over sorted positions
-        Mass3 mine{0.0, 0.0, 0.0};
+        Moment mine{0ull, 0ull, 0ull, 0ull, 0ull};
 #pragma unroll
         for (int k = 0; k < kTreeItems; ++k)
-            if (i0 + k < n) mine = m3_add(mine, body_m3(bd[k], sg ? ssg[i0 + k - base + 1] : 0, seg_root));
-        Mass3 tot;
-        Mass3 pre = m3_add(block_exclusive_scan3<kTreeThreads>(mine, &tot), bmass[blockIdx.x]);
+            if (i0 + k < n) mine = mo_add(mine, body_moment(bd[k], sg ? ssg[i0 + k - base + 1] : 0, seg_root));
+        Moment tot;
+        Moment pre = mo_add(block_exclusive_scan_mo<kTreeThreads>(mine, &tot), bmass[blockIdx.x]);
 #pragma unroll
         for (int k = 0; k < kTreeItems; ++k) {
             const int i = i0 + k;
             if (i < n) {
                 pmass[i] = pre;
-                pre = m3_add(pre, body_m3(bd[k], sg ? ssg[i - base + 1] : 0, seg_root));
+                pre = mo_add(pre, body_moment(bd[k], sg ? ssg[i - base + 1] : 0, seg_root));
                 if (i == n - 1) pmass[n] = pre;
             }
         }
@@ -447,7 +494,9 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
 
 // Cell records (skip, level, count) and the cell list for level bucketing, one thread per
 // cell (grid-stride over the S[n] cells). cell_rec[ord] holds the cell's first sorted
-// position on entry (written by k_prefix) and its record index on exit. The cells opened at
+// position on entry (written by k_prefix); on the fractional-mass path it holds the cell's
+// record index on exit (with cell_lvl, for the per-level summaries), on the integer-mass
+// path it is left as it was (the summary comes from the moments). The cells opened at
 // position i have levels lprev+1 .. lcp_i in ord order; each finds its extent end by a
 // galloping search over the keys (sorted within a graph, so a cell's bodies are contiguous).
 // One thread per cell keeps the long searches of the top levels, which all start at a
@@ -458,7 +507,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
                                                         const int* __restrict__ S, float4* __restrict__ rec,
                                                         int* __restrict__ rcount, int* __restrict__ cell_rec,
                                                         unsigned char* __restrict__ cell_lvl, int* __restrict__ rec_begin,
-                                                        DevState* st, const Mass3* __restrict__ pmass,
+                                                        DevState* st, const Moment* __restrict__ pmass,
                                                         const float4* __restrict__ seg_root, int* __restrict__ big_rc,
                                                         unsigned char* __restrict__ big_lvl) {
     __shared__ int hist[kMaxLevels + 1];
@@ -496,11 +545,13 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
         if (pmass) {
             // whole-number masses: a cell lighter than 2^24 has this exact float mass in any
             // summation order; heavier cells are summed in quadrant order later
-            const Mass3 a = pmass[i], z = pmass[b + 1];
-            const double M = z.m - a.m;
+            const Moment d = mo_sub(pmass[b + 1], pmass[i]);
+            const double M = (double)d.m;
             if (M < 16777216.0) {
                 const float4 c = seg_root[gi];
-                rec[rc] = make_float4((float)((z.x - a.x) / M + c.x), (float)((z.y - a.y) / M + c.y), (float)M,
+                const int sh = moment_shift(c.z);
+                rec[rc] = make_float4((float)(ldexp(i128_to_double(d.xl, d.xh) / M, -sh) + c.x),
+                                      (float)(ldexp(i128_to_double(d.yl, d.yh) / M, -sh) + c.y), (float)M,
                                       __uint_as_float(info));
             } else {
                 rec[rc].w = __uint_as_float(info);
@@ -700,8 +751,8 @@ void launch_tree(Bufs& b, hipStream_t s) {
     const int n = b.n;
     const int tiles = (n + kTreeTile - 1) / kTreeTile;
     const int* sg = b.nseg > 1 ? b.sg : nullptr;
-    Mass3* bm = b.int_mass ? b.bmass : nullptr;
-    Mass3* pm = b.int_mass ? b.pmass : nullptr;
+    Moment* bm = b.int_mass ? b.bmass : nullptr;
+    Moment* pm = b.int_mass ? b.pmass : nullptr;
     k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum, b.sort.vals[0], b.pos, b.mass,
                                          b.seg_root, bm, b.sbody);
     k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st, bm);

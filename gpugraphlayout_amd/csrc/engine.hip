@@ -236,6 +236,14 @@ int reset_state(gfa2_engine* h) {
 
 int build_groups(gfa2_engine* h);
 
+// The ABI status of the device error word (tree overflow first: it is the one a caller can
+// act on, with a larger cell_factor).
+int device_error(int err) {
+    if (err & kErrTreeOverflow) return GFA2_ERR_TREE_OVERFLOW;
+    if (err & kErrSortWait) return GFA2_ERR_DEVICE_WAIT;
+    return GFA2_OK;
+}
+
 // Copies DevState back (waits for the stream).
 int read_state(gfa2_engine* h, DevState* st) {
     GFA2_TRY(hipMemcpyAsync(st, h->b.st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
@@ -464,11 +472,9 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.sort.digit_hist, 8 * 256);
     GFA2_ALLOC(b.sort.status, (size_t)256 * sort_tiles);
     GFA2_ALLOC(b.sort.ticket, 1);
-    GFA2_ALLOC(b.sort.err, 1);
-    // look-back state starts empty: no published tile, ticket 0, no error
+    // look-back state starts empty: no published tile, ticket 0
     GFA2_TRY(hipMemsetAsync(b.sort.status, 0, sizeof(unsigned int) * 256 * (size_t)sort_tiles, h->stream));
     GFA2_TRY(hipMemsetAsync(b.sort.ticket, 0, sizeof(unsigned long long), h->stream));
-    GFA2_TRY(hipMemsetAsync(b.sort.err, 0, sizeof(int), h->stream));
     GFA2_ALLOC(b.lcp, n);
     GFA2_ALLOC(b.blocksum, tree_tiles);
     GFA2_ALLOC(b.S, (size_t)n + 1);
@@ -481,6 +487,9 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.level_list, maxcells);
     GFA2_ALLOC(b.bbox_part, kPartBlocks);
     GFA2_ALLOC(b.st, 1);
+    // a sort wait that runs out raises the instance's error word, so every later kernel of the
+    // iteration skips and gfa2_sync reports it (cleared by reset_state)
+    b.sort.err = &b.st->err;
     GFA2_ALLOC(b.node_off, (size_t)nseg + 1);
     GFA2_ALLOC(b.seg_root, nseg);
     GFA2_ALLOC(b.rec_begin, (size_t)nseg + 1);
@@ -712,8 +721,7 @@ int gfa2_sync(gfa2_handle h) {
     DevState st;
     rc = read_state(h, &st);
     if (rc != GFA2_OK) return rc;
-    if (st.err & kErrTreeOverflow) return GFA2_ERR_TREE_OVERFLOW;
-    return GFA2_OK;
+    return device_error(st.err);
 }
 
 int gfa2_get_forces(gfa2_handle h, int32_t kind, float* xy) {
@@ -756,7 +764,8 @@ int gfa2_get_tree(gfa2_handle h, int32_t cap, int32_t* ncells, int32_t* level, i
     DevState st;
     int rc = read_state(h, &st);
     if (rc != GFA2_OK) return rc;
-    if (st.err & kErrTreeOverflow) return GFA2_ERR_TREE_OVERFLOW;
+    rc = device_error(st.err);
+    if (rc != GFA2_OK) return rc;
     const int nrec = st.nrecords;
     std::vector<float4> rec((size_t)std::max(nrec, 1));
     std::vector<int> rcount((size_t)std::max(nrec, 1));
@@ -820,12 +829,6 @@ int gfa2_get_stats(gfa2_handle h, gfa2_stats* s) {
     s->root_y = st.rooty;
     s->radius = st.radius;
     s->error_flags = st.err;
-    if (h->b.sort.err) {
-        int sort_err = 0;
-        GFA2_TRY(hipMemcpyAsync(&sort_err, h->b.sort.err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-        GFA2_TRY(hipStreamSynchronize(h->stream));
-        if (sort_err) s->error_flags |= kErrSortWait;
-    }
     return GFA2_OK;
 }
 
@@ -986,13 +989,23 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     DevState* sst = nullptr;
     SegState* sseg = nullptr;
     float4* spart = nullptr;
-    GFA2_TRY(hipMalloc(&spos, sizeof(float2) * n));
-    GFA2_TRY(hipMalloc(&sfp, sizeof(float2) * n));
-    GFA2_TRY(hipMalloc(&sf, sizeof(float2) * n));
-    GFA2_TRY(hipMalloc(&sfga, sizeof(float2) * n));
-    GFA2_TRY(hipMalloc(&sst, sizeof(DevState)));
-    GFA2_TRY(hipMalloc(&sseg, sizeof(SegState) * b.nseg));
-    GFA2_TRY(hipMalloc(&spart, sizeof(float4) * kPartBlocks));
+    auto free_snapshot = [&]() {
+        for (void* p : {(void*)spos, (void*)sfp, (void*)sf, (void*)sfga, (void*)sst, (void*)sseg, (void*)spart})
+            if (p) (void)hipFree(p);
+    };
+    {
+        hipError_t ea = hipMalloc(&spos, sizeof(float2) * n);
+        if (ea == hipSuccess) ea = hipMalloc(&sfp, sizeof(float2) * n);
+        if (ea == hipSuccess) ea = hipMalloc(&sf, sizeof(float2) * n);
+        if (ea == hipSuccess) ea = hipMalloc(&sfga, sizeof(float2) * n);
+        if (ea == hipSuccess) ea = hipMalloc(&sst, sizeof(DevState));
+        if (ea == hipSuccess) ea = hipMalloc(&sseg, sizeof(SegState) * b.nseg);
+        if (ea == hipSuccess) ea = hipMalloc(&spart, sizeof(float4) * kPartBlocks);
+        if (ea != hipSuccess) {
+            free_snapshot();
+            return hip_status(ea);
+        }
+    }
     auto copy_all = [&](bool save) -> hipError_t {
         hipError_t e = hipSuccess;
         auto cp = [&](void* dst, const void* src, size_t bytes) {
@@ -1078,8 +1091,7 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     }
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
-    for (void* p : {(void*)spos, (void*)sfp, (void*)sf, (void*)sfga, (void*)sst, (void*)sseg, (void*)spart})
-        (void)hipFree(p);
+    free_snapshot();
     if (e != hipSuccess) {
         h->tune = keep;
         apply_tune(h);
@@ -1261,6 +1273,7 @@ const char* gfa2_strerror(int code) {
         case GFA2_ERR_HIP: return "HIP runtime error";
         case GFA2_ERR_NO_GRAPH: return "no graph loaded";
         case GFA2_ERR_UNSUPPORTED: return "unsupported configuration";
+        case GFA2_ERR_DEVICE_WAIT: return "a bounded device wait ran out (the iteration was skipped)";
         default: return "unknown error";
     }
 }

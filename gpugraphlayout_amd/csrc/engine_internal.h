@@ -42,7 +42,6 @@ struct DevState {
     float global_speed, speed_efficiency, jitter_tolerance, k_s_max;
     double swing, traction;
     unsigned long long interactions;
-    unsigned long long interactions_total;   // unused: kept per graph in SegState
     int ncells;
     int nrecords;
     int max_level;      // deepest cell level
@@ -56,10 +55,16 @@ struct DevState {
     int level_cursor[kMaxLevels + 1];
 };
 
-// Prefix sums over sorted bodies of mass and mass-weighted position (relative to the
-// graph's root centre), for the integer-mass summary path.
-struct Mass3 {
-    double m, x, y;
+// Prefix sums over sorted bodies for the integer-mass summary path: mass, and mass times the
+// position relative to the graph's root centre in fixed point (coordinate * 2^s, with s per
+// graph from its root radius so every coordinate is below 2^62 in magnitude). All sums are
+// integers modulo 2^128 (two 64-bit limbs), so a cell's moment, the difference of two prefix
+// sums, is exact: it does not depend on the bodies before the cell, on the other graphs of a
+// batch or on how the sums were tiled.
+struct Moment {
+    unsigned long long xl, xh;   // sum m * qx, low and high limb
+    unsigned long long yl, yh;   // sum m * qy
+    unsigned long long m;        // sum m
 };
 constexpr int kMaxBig = 2048;   // cells too heavy for an exact prefix-sum mass (>= 2^24)
 
@@ -77,7 +82,8 @@ struct SortBuffers {
     unsigned int* digit_hist = nullptr;   // 8 x 256: digit totals of every pass
     unsigned int* status = nullptr;       // tiles x 256: published tile counts / prefixes
     unsigned long long* ticket = nullptr; // tile tickets, never reset while the tile count holds
-    int* err = nullptr;                   // set if a look-back wait ran out (never expected)
+    int* err = nullptr;                   // the instance's error word: kErrSortWait is raised there if a
+                                          // look-back wait runs out (never expected)
 };
 
 // FA2 scalar parameters (src/RPForceAtlas2.cpp:28-55 defaults).
@@ -168,8 +174,8 @@ struct Bufs {
     // the same float mass in every summation order, so it is summarised from prefix sums;
     // only heavier cells are summed in quadrant order.
     bool int_mass = false;
-    Mass3* pmass = nullptr;           // n + 1
-    Mass3* bmass = nullptr;           // per tree tile
+    Moment* pmass = nullptr;          // n + 1
+    Moment* bmass = nullptr;          // per tree tile
     int* big_rc = nullptr;            // kMaxBig
     unsigned char* big_lvl = nullptr; // kMaxBig
 };

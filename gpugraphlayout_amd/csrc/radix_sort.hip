@@ -377,7 +377,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
                 __builtin_amdgcn_s_sleep(1);
             }
             if (spins >= kMaxSpins) {
-                atomicOr(err, 1);
+                atomicOr(err, kErrSortWait);
                 break;
             }
             excl += v & kStatCount;

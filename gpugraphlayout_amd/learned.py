@@ -169,6 +169,113 @@ def learned_run(m, e, plain, start_scale, feature_seed, feature_steps, check):
     return r
 
 
+def multistart_run(m, e, plain, start_scale, feature_seeds, feature_steps, per_start, check, random_seeds=None):
+    """Several starts in turn on the same engine, each refined by FA2 for up to `per_start`
+    iterations with a stress check every `check` iterations, until one reaches NS <= the
+    plain run's NS or the plain run's iteration budget is spent. The starts are the network's
+    proposals for the given feature seeds (the features' random-walk fields differ per seed),
+    or, with random_seeds, uniform random starts (the same protocol without the network).
+    Timed as run: every start, inference, FA2 iteration and stress check is inside the
+    measured wall time."""
+    ns_plain, budget = plain["ns_plain"], plain["plain_iterations"]
+    tries = random_seeds if random_seeds is not None else feature_seeds
+    e.sync()
+    t0 = time.perf_counter()
+    spent, reached, used, ns = 0, None, 0, None
+    for j, sd in enumerate(tries):
+        if spent >= budget:
+            break
+        e.reset()
+        if random_seeds is not None:
+            e.set_positions(start_positions_n(e.n, sd))
+        else:
+            gid = m.add_engine_graph(e, feature_seed=sd, steps=feature_steps)
+            m.infer_to_engine(gid, start_scale, e)
+            m.release_graph(gid)
+        used = j + 1
+        it = 0
+        ns = e.normalized_stress()[0]
+        while ns > ns_plain and it < per_start and spent < budget:
+            step = min(check, per_start - it, budget - spent)
+            e.step(step)
+            it += step
+            spent += step
+            ns = e.normalized_stress()[0]
+        if ns <= ns_plain:
+            reached = spent
+            break
+    e.sync()
+    t = time.perf_counter() - t0
+    out = {"starts_used": used, "iterations": spent, "reached": reached is not None, "seconds": t,
+           "time_ratio": t / plain["plain_seconds"], "ns_end": ns}
+    return out
+
+
+def start_positions_n(n, seed):
+    rng = np.random.default_rng(seed)
+    side = 2 * np.sqrt(n)
+    return ((rng.random((n, 2)) - 0.5) * side).astype(np.float32)
+
+
+def save_model(m, path, meta):
+    """Parameters (FP32 masters, .npy) and a JSON sidecar with the settings they were trained
+    with and the training targets' mean RMS radius."""
+    np.save(path, m.params())
+    meta = dict(meta, target_scale=m.target_scale(), num_params=int(m.nparams))
+    with open(path[:-4] + ".json" if path.endswith(".npy") else path + ".json", "w") as fh:
+        json.dump(meta, fh, indent=1)
+
+
+def load_model(path, device=0):
+    """(Mpnn with the saved parameters, metadata dict)."""
+    with open(path[:-4] + ".json" if path.endswith(".npy") else path + ".json") as fh:
+        meta = json.load(fh)
+    p = np.load(path, allow_pickle=False)
+    m = Mpnn(seed=0, device=device)
+    if len(p) != m.nparams:
+        m.close()
+        raise ValueError(f"{path}: {len(p)} parameters, the network has {m.nparams}")
+    m.set_params(p)
+    return m, meta
+
+
+def evaluate_sbm(m, target_scale, seeds, kappa, feature_steps, check, iters_plain=1000, multistart=None, log=None):
+    """Time to equal stress on held-out SBM graphs: plain FA2 (random start, `iters_plain`
+    iterations) against the learned start (single start, the protocol of learned_run) and,
+    with multistart = (starts, per_start), several learned proposals and, for comparison,
+    as many random starts under the same protocol."""
+    out = []
+    for seed in seeds:
+        g = sbm_graph(seed)
+        e = Engine(g.n, g.src, g.dst, **FA2)
+        e.set_graph_capture(True)
+        e.stress_setup(1024, 977, seed=seed)
+        plain = plain_run(e, start_positions(g, 10_000 + seed), iters_plain)
+        r = learned_run(m, e, plain, kappa * target_scale, 10_000 + seed, feature_steps, check)
+        r.pop("learned_trajectory", None)
+        r.update({"seed": seed, "nodes": g.n, "edges": g.m})
+        if multistart:
+            starts, per_start = multistart
+            fseeds = [10_000 + seed + 7919 * j for j in range(starts)]
+            r["multistart_learned"] = multistart_run(m, e, plain, kappa * target_scale, fseeds, feature_steps,
+                                                     per_start, check)
+            rseeds = [20_000 + seed + 7919 * j for j in range(starts)]
+            r["multistart_random"] = multistart_run(m, e, plain, None, None, feature_steps, per_start, check,
+                                                    random_seeds=rseeds)
+        e.close()
+        out.append(r)
+        if log:
+            log(json.dumps({k: v for k, v in r.items()}))
+    return out
+
+
+def summarise_multistart(results, key):
+    rs = [r[key] for r in results if key in r]
+    ok = [x["time_ratio"] for x in rs if x["reached"]]
+    return {"graphs": len(rs), "reached": len(ok), "within_half": sum(1 for x in ok if x <= 0.5),
+            "median_time_ratio": float(np.median(ok)) if ok else None}
+
+
 def rmat_engine(g):
     """Engine for an R-MAT test graph with its stress pairs, and the reference's glibc start."""
     e = Engine(g.n, g.src, g.dst, **FA2_RMAT)
@@ -200,6 +307,18 @@ def run_sbm(args, log):
         models[fs] = train(train_gs, targets, args.steps, lr=args.lr, feature_steps=fs, log=log)
         train_seconds[fs] = time.perf_counter() - t2
     del train_gs, targets
+    if args.save_params and len(models) == 1:
+        fs = next(iter(models))
+        save_model(models[fs], args.save_params,
+                   {"workload": "sbm50k", "train_seeds": [1000, 1000 + args.train - 1], "nodes": args.nodes,
+                    "target_iterations": args.target_iters, "train_steps": args.steps, "lr": args.lr,
+                    "feature_steps": fs, "kappa": args.kappa, "check": args.check,
+                    "command": "python -m gpugraphlayout_amd.learned " + " ".join(sys.argv[1:])})
+        log(f"saved {args.save_params}")
+    if args.test == 0:
+        for m in models.values():
+            m.close()
+        return {"workload": "sbm50k", "train_graphs": args.train, "train_seconds": train_seconds}
     # validation graphs come from the training seed range past the graphs trained on
     seed0 = 2000 if args.validate else 5000
     results = []
@@ -316,6 +435,12 @@ def main(argv=None):
     ap.add_argument("--sweep-kappa", default="0.25,0.5,1.0")
     ap.add_argument("--inference-timing", type=int, default=0,
                     help="rmat: time features + inference on R-MAT 2^24 this many times, then exit")
+    ap.add_argument("--save-params", default=None, help="sbm50k: save the trained network here (.npy + .json)")
+    ap.add_argument("--evaluate", default=None, help="sbm50k: evaluate a saved network on held-out graphs")
+    ap.add_argument("--first-seed", type=int, default=5000, help="with --evaluate: first held-out seed")
+    ap.add_argument("--starts", type=int, default=1, help="with --evaluate: learned proposals per graph (> 1: "
+                                                          "also the multi-start protocol, and random starts under it)")
+    ap.add_argument("--per-start", type=int, default=100, help="with --starts: FA2 iterations per proposal")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     d = DEFAULTS[args.workload]
@@ -328,6 +453,19 @@ def main(argv=None):
 
     if args.inference_timing:
         summary = inference_timing(args.inference_timing, log)
+    elif args.evaluate:
+        m, meta = load_model(args.evaluate)
+        seeds = list(range(args.first_seed, args.first_seed + args.test))
+        ms = (args.starts, args.per_start) if args.starts > 1 else None
+        res = evaluate_sbm(m, meta["target_scale"], seeds, meta["kappa"], meta["feature_steps"], meta["check"],
+                           multistart=ms, log=log)
+        m.close()
+        summary = {"workload": "sbm50k", "model": args.evaluate, "first_seed": args.first_seed, "test_graphs": len(seeds),
+                   "single_start": summarise(res), "results": res}
+        if ms:
+            summary["multistart"] = {"starts": args.starts, "per_start": args.per_start,
+                                     "learned": summarise_multistart(res, "multistart_learned"),
+                                     "random": summarise_multistart(res, "multistart_random")}
     else:
         summary = run_sbm(args, log) if args.workload == "sbm50k" else run_rmat(args, log)
     print(json.dumps({k: v for k, v in summary.items() if k != "results"}))

@@ -27,6 +27,8 @@ extern "C" {
 #define GFA2_ERR_HIP (-6)            /* any other HIP runtime error, src/RPCommon.hpp:33-41 exits */
 #define GFA2_ERR_NO_GRAPH (-7)       /* step/positions before set_graph */
 #define GFA2_ERR_UNSUPPORTED (-8)    /* e.g. exact mode: src/graph_viewer.cpp:93-97 */
+#define GFA2_ERR_DEVICE_WAIT (-9)    /* a bounded wait of a device kernel ran out (the iteration's results
+                                        are invalid and its later kernels were skipped); not expected */
 
 typedef struct gfa2_engine* gfa2_handle;
 
@@ -69,7 +71,7 @@ typedef struct gfa2_stats {
     int32_t cells;              /* quadtree cells of the last tree */
     int32_t max_level;          /* deepest cell level (root 0); max body depth = max_level + 1 */
     float root_x, root_y, radius;
-    int32_t error_flags;        /* bit 0: tree capacity exceeded; bit 1: a sort wait ran out */
+    int32_t error_flags;        /* bit 0: tree capacity exceeded; bit 1: a device wait ran out */
 } gfa2_stats;
 
 /* Force components for gfa2_get_forces. */
@@ -104,9 +106,8 @@ int gfa2_set_graph_coo(gfa2_handle h, int32_t n, int64_t m, const int32_t* src, 
 /* A batch of independent graphs laid out together (no counterpart in the reference, which
  * holds one graph per process): graph g owns nodes [node_off[g], node_off[g+1]) of a
  * block-diagonal symmetric CSR (no edge may leave its graph). Each graph gets its own root
- * box, tree, walk groups and speed state, so it follows the trajectory it would follow
- * alone: the same tree, masses and counts, with cell centroids (FP64 prefix sums over the
- * batch, for integer masses) equal to the last float bit. Positions are the concatenation. */
+ * box, tree, walk groups and speed state, so it follows, bit for bit, the trajectory it would
+ * follow alone. Positions are the concatenation. */
 int gfa2_set_graph_batch(gfa2_handle h, int32_t ngraphs, const int32_t* node_off, const int64_t* rowptr,
                          const int32_t* col, const float* mass);
 /* Per-graph speed, efficiency, swing, traction and interaction count of the last
@@ -131,8 +132,10 @@ int gfa2_reset(gfa2_handle h);
  * iteration. */
 int gfa2_step(gfa2_handle h, int32_t iters);
 
-/* Wait for queued work; reports device-side errors (tree overflow). The reference's
- * cudaDeviceSynchronize at src/RPGPUForceAtlas2.cu:246. */
+/* Wait for queued work; reports device-side errors (GFA2_ERR_TREE_OVERFLOW,
+ * GFA2_ERR_DEVICE_WAIT). The reference's cudaDeviceSynchronize at src/RPGPUForceAtlas2.cu:246.
+ * The error stays raised (and every later iteration is skipped) until gfa2_reset or
+ * gfa2_set_graph. */
 int gfa2_sync(gfa2_handle h);
 
 /* Force vectors of the last iteration (interleaved x,y). */

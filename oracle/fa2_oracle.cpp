@@ -31,6 +31,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <thread>
 
 namespace {
 
@@ -213,15 +214,21 @@ void thresholds(float radius, float itolsq, float epssq, int levels, std::vector
 // at stack depth d (children of the root are at d = 0) is accepted only when every
 // lane of the group has dx^2+dy^2+epssq >= dq[d] (:661); otherwise it is opened.
 // Accumulation: a += k_r*dx*m_i*m_n/tmp, in that operation order (:654-665).
+// starts: the first sorted position of each group to walk (nstarts of them), or null for all
+// groups (0, G, 2G, ...); only the walked groups' bodies get their rep entries written.
 void walk(const Tree& t, int n, const float* pos, const float* mass, const std::vector<int>& perm,
-          float k_r, float itolsq, float epssq, int G, float* rep, long long* interactions) {
+          float k_r, float itolsq, float epssq, int G, float* rep, long long* interactions,
+          const int* starts = nullptr, int nstarts = 0) {
     std::vector<float> dq;
     thresholds(t.radius, itolsq, epssq, 128, dq);
     long long inter = 0;
     std::vector<float> ax(G), ay(G), px(G), py(G), mi(G);
     struct Frame { int cell, q; };
     std::vector<Frame> stack;
-    for (int g0 = 0; g0 < n; g0 += G) {
+    const int ngroups = starts ? nstarts : (n + G - 1) / G;
+    for (int gi = 0; gi < ngroups; ++gi) {
+        const int g0 = starts ? starts[gi] : gi * G;
+        if (g0 < 0 || g0 >= n) continue;
         int L = std::min(G, n - g0);
         for (int l = 0; l < L; ++l) {
             int b = perm[g0 + l];
@@ -378,6 +385,22 @@ int orc_bh_repulsion(int n, float* pos, const float* mass, float k_r, float itol
     return t.max_depth;
 }
 
+// The same repulsion for the listed groups only (each a first sorted position, a multiple of
+// G): for checks at sizes where walking every group on one core takes too long. rep (2n
+// floats) gets the walked groups' bodies only; interactions counts those groups'.
+int orc_bh_repulsion_groups(int n, float* pos, const float* mass, float k_r, float itolsq, float epssq,
+                            int G, int max_levels, int nstarts, const int* starts, float* rep, int* perm,
+                            long long* interactions) {
+    Tree t;
+    if (build_tree(n, pos, t, max_levels) != 0) return -1;
+    summarize(t, 0, pos, mass);
+    std::vector<int> p, pre;
+    order(t, 0, p, pre);
+    if (perm) for (int i = 0; i < n; ++i) perm[i] = p[i];
+    walk(t, n, pos, mass, p, k_r, itolsq, epssq, G, rep, interactions, starts, nstarts);
+    return t.max_depth;
+}
+
 // SpeedKernel, src/RPFA2Kernels.cu:111-221: S = sum m*|F-F'|, T = sum m*|F+F'|/2, summed
 // in double here (the reference's block-tree order is not reproducible). state =
 // {global_speed, speed_efficiency, jitter_tolerance, k_s_max}; the update restates
@@ -482,21 +505,32 @@ int orc_fa2_step(int n, long long m, const int* src, const int* dst, const float
 
 // FP64 exact pairwise repulsion with the same softening: for each sampled body i,
 // k_r * m_i * sum_j m_j (p_i - p_j) / (|p_i - p_j|^2 + epssq). Self terms are 0.
+// The sample is split over up to `threads` host threads; each sum runs over j in ascending
+// order in one thread, so the result does not depend on the thread count.
 void orc_exact_repulsion_f64(int n, const float* pos, const float* mass, double k_r, double epssq,
-                             int nsample, const int* sample, double* out) {
-    for (int s = 0; s < nsample; ++s) {
-        int i = sample[s];
-        double xi = pos[2 * i], yi = pos[2 * i + 1], mi = mass[i];
-        double ax = 0.0, ay = 0.0;
-        for (int j = 0; j < n; ++j) {
-            double dx = xi - (double)pos[2 * j], dy = yi - (double)pos[2 * j + 1];
-            double w = (double)mass[j] / (dx * dx + dy * dy + epssq);
-            ax += dx * w;
-            ay += dy * w;
+                             int nsample, const int* sample, double* out, int threads) {
+    auto run = [=](int s0, int s1) {
+        for (int s = s0; s < s1; ++s) {
+            int i = sample[s];
+            double xi = pos[2 * i], yi = pos[2 * i + 1], mi = mass[i];
+            double ax = 0.0, ay = 0.0;
+            for (int j = 0; j < n; ++j) {
+                double dx = xi - (double)pos[2 * j], dy = yi - (double)pos[2 * j + 1];
+                double w = (double)mass[j] / (dx * dx + dy * dy + epssq);
+                ax += dx * w;
+                ay += dy * w;
+            }
+            out[2 * s] = k_r * mi * ax;
+            out[2 * s + 1] = k_r * mi * ay;
         }
-        out[2 * s] = k_r * mi * ax;
-        out[2 * s + 1] = k_r * mi * ay;
-    }
+    };
+    if (threads < 1) threads = 1;
+    if (threads > nsample) threads = nsample > 0 ? nsample : 1;
+    std::vector<std::thread> pool;
+    for (int k = 1; k < threads; ++k)
+        pool.emplace_back(run, (int)((long long)nsample * k / threads), (int)((long long)nsample * (k + 1) / threads));
+    run(0, (int)((long long)nsample / threads));
+    for (std::thread& th : pool) th.join();
 }
 
 }  // extern "C"

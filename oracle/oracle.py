@@ -37,6 +37,7 @@ def lib():
             build()
         L = ctypes.CDLL(LIB)
         L.orc_bh_repulsion.restype = ctypes.c_int
+        L.orc_bh_repulsion_groups.restype = ctypes.c_int
         L.orc_tree.restype = ctypes.c_int
         L.orc_fa2_step.restype = ctypes.c_int
         L.orc_cpu_create.restype = ctypes.c_void_p
@@ -140,6 +141,24 @@ def bh_repulsion(pos, mass, k_r, theta=1.0, epssq=None, group=32, max_levels=128
     return rep, {"perm": perm, "max_depth": md, "interactions": inter.value}
 
 
+def bh_repulsion_groups(pos, mass, k_r, starts, theta=1.0, epssq=None, group=32, max_levels=128):
+    """bh_repulsion for the groups starting at the given sorted positions only. Returns the
+    repulsion (zero outside those groups), the permutation and the groups' interaction count."""
+    n = len(mass)
+    pos = f32(pos).copy()
+    rep = np.zeros((n, 2), np.float32)
+    perm = np.zeros(n, np.int32)
+    inter = ctypes.c_longlong(0)
+    eps = np.float32(0.05 * 0.05) if epssq is None else np.float32(epssq)
+    itolsq = np.float32(1.0) / (np.float32(theta) * np.float32(theta))
+    starts = i32(starts)
+    md = lib().orc_bh_repulsion_groups(n, _p(pos), _p(f32(mass)), F(k_r), F(itolsq), F(eps), group, max_levels,
+                                       len(starts), _p(starts), _p(rep), _p(perm), ctypes.byref(inter))
+    if md < 0:
+        raise RuntimeError("oracle tree exceeded max_levels")
+    return rep, {"perm": perm, "max_depth": md, "interactions": inter.value}
+
+
 def speed(f, fprev, mass, state):
     st = f32(state).copy()
     out = np.zeros(2, np.float64)
@@ -182,12 +201,14 @@ def fa2_step(pos, fprev, state, src, dst, mass, strong, k_g, k_r, theta=1.0, eps
                             "interactions": inter.value, "max_depth": md}
 
 
-def exact_repulsion_f64(pos, mass, k_r, sample, epssq=None):
+def exact_repulsion_f64(pos, mass, k_r, sample, epssq=None, threads=None):
     eps = float(np.float32(0.05 * 0.05)) if epssq is None else float(epssq)
     sample = i32(sample)
     out = np.zeros((len(sample), 2), np.float64)
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
     lib().orc_exact_repulsion_f64(len(mass), _p(f32(pos)), _p(f32(mass)), ctypes.c_double(k_r), ctypes.c_double(eps),
-                                  len(sample), _p(sample), _p(out))
+                                  len(sample), _p(sample), _p(out), int(threads))
     return out
 
 

@@ -218,6 +218,28 @@ def test_walk_group_vote_is_conservative(oracle):
     assert max(errs) < 0.05
 
 
+def test_walk_of_listed_groups_equals_full_walk(oracle):
+    """Walking only some groups (the large-graph check) gives those groups' bodies exactly
+    the full walk's forces, and nothing to the others."""
+    pos, mass = rand_case(3000, 24)
+    full, info = oracle.bh_repulsion(pos, mass, 2.0, group=32)
+    starts = np.array([0, 32 * 7, 32 * 50, 32 * 93], np.int32)
+    part, pinfo = oracle.bh_repulsion_groups(pos, mass, 2.0, starts, group=32)
+    ids = np.concatenate([info["perm"][s:s + 32] for s in starts])
+    np.testing.assert_array_equal(pinfo["perm"], info["perm"])
+    np.testing.assert_array_equal(part[ids].view(np.uint32), full[ids].view(np.uint32))
+    rest = np.setdiff1d(np.arange(3000), ids)
+    assert not np.any(part[rest])
+    assert 0 < pinfo["interactions"] < info["interactions"]
+
+
+def test_exact_pairwise_does_not_depend_on_threads(oracle):
+    pos, mass = rand_case(500, 25)
+    a = oracle.exact_repulsion_f64(pos, mass, 1.0, np.arange(37), threads=1)
+    b = oracle.exact_repulsion_f64(pos, mass, 1.0, np.arange(37), threads=5)
+    np.testing.assert_array_equal(a, b)
+
+
 def test_walk_symmetric_square(oracle):
     pos = np.array([[-1, -1], [1, -1], [-1, 1], [1, 1]], F32)
     rep, _ = oracle.bh_repulsion(pos, np.full(4, 2, F32), 1.0, theta=1e-6)
