@@ -126,7 +126,7 @@ def timed_run(eng, args, steps):
     """Autotune (optional), then `steps` timed iterations with per-group HIP events."""
     tune_s = None
     if args.walk_block > 0:
-        eng.set_launch_config(args.walk_block, args.blocks_per_cu, args.walk_fetch)
+        eng.set_launch_config(args.walk_block, args.blocks_per_cu)
     launch = eng.launch_config()
     if args.autotune > 0 and args.walk_block <= 0:
         t0 = time.perf_counter()
@@ -234,8 +234,7 @@ def run_single(args):
 
     out = base_record(args, steps, dt, value,
                       dict(cfg, group_size=args.group, parallelism="single-gpu",
-                           walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"],
-                           walk_fetch=launch["walk_fetch"]))
+                           walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"]))
     out.update({
         "roofline": walk_roofline(groups, s0, s1, steps),
         "cpu_baseline": cpu,
@@ -327,8 +326,7 @@ def run_batch(args):
     cfg = {"workload": "barabasi-albert-4096-graphs", "graphs": len(items), "n": "round(exp(U[ln100, ln20000]))",
            "m": "U{2..8}", "seed": 2, "nodes": V, "edges": E, "gravity": "strong", "k_g": k_g, "k_r": k_r,
            "theta": 1.0, "init": "uniform square of side 2*sqrt(n) per graph", "group_size": args.group,
-           "parallelism": "single-gpu", "walk_block": launch["walk_block"], "blocks_per_cu": launch["blocks_per_cu"],
-           "walk_fetch": launch["walk_fetch"]}
+           "parallelism": "single-gpu", "walk_block": launch["walk_block"], "blocks_per_cu": launch["blocks_per_cu"]}
     out = base_record(args, steps, dt, value, cfg)
     out.update({
         "roofline": walk_roofline(groups, s0, s1, steps),
@@ -360,8 +358,6 @@ def main():
                     help="iterations per candidate launch shape before timing (0: default shapes)")
     ap.add_argument("--walk-block", type=int, default=0, help="fixed walk block size (skips the autotuner)")
     ap.add_argument("--blocks-per-cu", type=int, default=8, help="with --walk-block: grid-stride blocks per CU")
-    ap.add_argument("--walk-fetch", type=int, default=0,
-                    help="with --walk-block: record fetch mode (0 direct, 1 window, 2 pairs)")
     ap.add_argument("--graph-replay", type=int, default=100,
                     help="extra iterations replayed as one captured hipGraph per iteration (reported apart)")
     ap.add_argument("--batch-sample", type=int, default=64, help="graphs laid out alone for the sequential estimate")

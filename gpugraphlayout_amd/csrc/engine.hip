@@ -949,7 +949,7 @@ int gfa2_set_launch_config(gfa2_handle h, const gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     const int bt = c->walk_block;
     if ((bt != 64 && bt != 128 && bt != 256 && bt != 512) || c->blocks_per_cu < 1 || c->blocks_per_cu > 64 ||
-        c->walk_fetch < 0 || c->walk_fetch > 2)
+        c->walk_fetch != 0)
         return GFA2_ERR_INVALID_ARG;
     h->tune = LaunchConfig{bt, c->blocks_per_cu, c->walk_fetch};
     apply_tune(h);
@@ -1062,13 +1062,11 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
             win = cand;
         }
     };
-    // Stage 1: the walk's block size and record window (the walk dominates); stage 2: the
-    // grid of the grid-stride kernels, with the stage-1 winner.
+    // Stage 1: the walk's block size (the walk dominates); stage 2: the grid of the
+    // grid-stride kernels, with the stage-1 winner.
     const int bts[] = {64, 128, 256, 512};
-    const int fetches[] = {0, 1, 2};
     const int bpcs[] = {1, 2, 4, 8};
-    for (int fetch : fetches)
-        for (int bt : bts) trial(LaunchConfig{bt, keep.blocks_per_cu, fetch});
+    for (int bt : bts) trial(LaunchConfig{bt, keep.blocks_per_cu, 0});
     const LaunchConfig shape = win;
     for (int bpc : bpcs)
         if (bpc != shape.blocks_per_cu) trial(LaunchConfig{shape.walk_bt, bpc, shape.walk_fetch});

@@ -95,14 +95,14 @@ struct Params {
     int exact = 0;          // 1: exact O(V^2) repulsion instead of the tree and walk
     int linlog = 0;         // attraction log(1+d)/d per unit vector
     int walk_bt = 256;      // walk threads per block (64..512); results do not depend on it
-    int walk_fetch = 0;     // walk record fetch mode (0 direct, 1 window, 2 paired); no result depends on it
+    int walk_fetch = 0;     // reserved (0)
 };
 
 // Launch shapes the autotuner chooses from; none of them changes any result bit.
 struct LaunchConfig {
     int walk_bt;            // 64, 128, 256, 512
     int blocks_per_cu;      // grid of the grid-stride kernels (keys, displacement, bbox)
-    int walk_fetch;         // 0: one record load per step; 1: group-wide windows; 2: records in pairs
+    int walk_fetch;         // reserved (0)
 };
 
 constexpr int kAttrChunk = 4096;    // entries per block for rows over 2,048 entries

@@ -181,26 +181,7 @@ __global__ __launch_bounds__(256) void k_attract_heavy(const float2* __restrict_
     fga[v] = make_float2(g.x + ax, g.y + ay);
 }
 
-// ---------------------------------------------------------------------------------------
-// Barnes-Hut repulsion walk. Reference: ForceCalculationKernel, src/RPBHKernels.cu:566-693.
-//
-// Groups of G consecutive bodies of a graph's sorted (in-order) permutation walk that
-// graph's tree together; groups start at the graph's first sorted position, as the
-// reference's warps start at position 0 (a batch of graphs is laid out exactly as each
-// graph alone). G = 32 reproduces the reference's 32-lane warp vote on wave64 by giving
-// each half-wave its own group; G = 64 votes over the whole wave. The tree is the
-// pre-order record array, so the walk needs no stack: from record r, a body is always
-// taken directly (r+1); a cell is taken as a whole when every active lane of the group
-// has dx^2+dy^2+epssq >= dq[level-1] (r += skip), and otherwise opened (r+1). That visits
-// children in quadrant order and accumulates in the reference's DFS order.
-// dq[d] = (2*radius)^2*itolsq*0.25^d + epssq (src/RPBHKernels.cu:576-590) is formed with
-// ldexp, which is exact for the power-of-4 scaling.
-// The term k_r*m_i*m_n*dx/(d^2+eps^2) uses a hardware reciprocal (the reference divides);
-// the difference is an ulp or two per term.
-// The epilogue adds gravity + attraction, writes the total force, and forms the group's
-// swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
-// ---------------------------------------------------------------------------------------
-// The end of a body's force evaluation, shared by the walk and the exact kernel: total force
+// The end of a body's force evaluation in the exact kernel: total force
 // F = F_ga + F_r, optional repulsion capture, and the group's swing/traction/interaction sums
 // (SpeedKernel's per-body terms, src/RPFA2Kernels.cu:111-161), reduced within the G lanes.
 template <int G>
@@ -250,49 +231,57 @@ __device__ __forceinline__ int xcd_block() {
     return (xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1)) + (blockIdx.x >> 3);
 }
 
-// One walk step on record c for the group (G lanes): the vote, the accumulation and the
-// advance of the record index (1, or the subtree size of an accepted cell). A step that is
-// not live contributes nothing: fma(d, 0, a) == a, since a is never -0 here.
-template <int G>
-__device__ __forceinline__ int walk_step(const float4 c, bool live, unsigned long long act, float px, float py,
-                                         float kmi, float base, float epssq, float& ax, float& ay,
-                                         unsigned int& inter) {
-    const unsigned int info = __float_as_uint(c.w);
-    const float dx = px - c.x;
-    const float dy = py - c.y;
-    const float d2 = dx * dx + dy * dy + epssq;
-    const float thr = ldexpf(base, -2 * (int)(info & 31u)) + epssq;
-    // Lane masks from plain compares (each lands in a scalar register pair), combined with
-    // scalar operations: bodies, lanes closer than the threshold, live lanes.
-    const unsigned long long body = __builtin_amdgcn_ballot_w64(info == 0xffffffffu);
-    const unsigned long long near = __builtin_amdgcn_ballot_w64(!(d2 >= thr));
-    const unsigned long long livem = __builtin_amdgcn_ballot_w64(live);
-    const unsigned long long fails = near & ~body & act & livem;
-    // The group's vote (each half-wave is one group when G == 32).
-    unsigned long long okm;
-    if constexpr (G == 32)
-        okm = (((unsigned int)fails == 0u) ? 0x00000000ffffffffull : 0ull) |
-              (((fails & 0xffffffff00000000ull) == 0ull) ? 0xffffffff00000000ull : 0ull);
-    else
-        okm = fails == 0ull ? ~0ull : 0ull;
-    const unsigned long long accm = (body | okm) & livem;
-    const bool accept = __builtin_amdgcn_inverse_ballot_w64(accm);
-    const bool jump = __builtin_amdgcn_inverse_ballot_w64(accm & ~body);
-    inter += __builtin_amdgcn_inverse_ballot_w64(accm & act) ? 1u : 0u;
-    // Branch-free: a rejected cell adds fma(d, 0, a) == a (a is never -0 here).
-    const float w = accept ? kmi * c.z * __builtin_amdgcn_rcpf(d2) : 0.0f;
-    ax = fmaf(dx, w, ax);
-    ay = fmaf(dy, w, ay);
-    return jump ? max((int)(info >> kInfoLevelBits), 1) : 1;
+// ---------------------------------------------------------------------------------------
+// Barnes-Hut repulsion walk. Reference: ForceCalculationKernel, src/RPBHKernels.cu:566-693.
+//
+// Groups of G consecutive bodies of a graph's sorted (in-order) permutation walk that graph's
+// pre-order record array together; groups start at the graph's first sorted position, as the
+// reference's warps start at position 0 (a graph in a batch is walked as it is alone). G = 32
+// reproduces the reference's 32-lane warp vote; G = 64 votes over the whole wave. One
+// wavefront walks one group in two parts:
+//
+//  traversal  records are tested 64 at a time, one per lane, taken from a stack of pending
+//             {record, end of its parent's subtree} entries in LDS. A body record is always
+//             accepted. A cell at level L is accepted when every active body b of the group
+//             has d2(b, c) >= dq[L-1] (the reference's all-lanes vote, :661), with
+//             d2 = fma(dx, dx, fma(dy, dy, eps)) and dq[d] = (2 radius)^2 itolsq 0.25^d + eps
+//             (:576-590, formed with ldexp, exact for the power-of-4 scaling). Most cells are
+//             decided without the bodies one by one: d2 is monotone in |dx| and |dy|, so the
+//             distance from the cell's centroid to the group's bounding box gives a float lower
+//             bound of every body's d2 (accept when it clears dq), and the first, middle or
+//             last body's own d2 below dq rejects. Only the cells left are voted on, body by
+//             body, all lanes at once (lanes = bodies, as in the reference). An accepted record
+//             is appended {x, y, m} to an LDS list; an opened cell pushes its first child
+//             {r + 1, r + skip}; every tested record pushes its next sibling {r + size, end}
+//             while it lies inside its parent. The children go on top, so the traversal runs
+//             depth-first in blocks of up to 64 records (one record per round near the stack's
+//             capacity, where a depth-first walk adds at most one entry per tree level).
+//  sums       when the list is nearly full, and at the end, every body adds the list's terms
+//             k_r m_i m_n d / d2 (hardware reciprocal; the reference divides, an ulp or two per
+//             term); lanes 0-31 take the even entries and lanes 32-63 the odd ones when G = 32,
+//             added together at the end.
+// The accepted set is the reference's; the order of the terms is the traversal's (fixed for a
+// given tree, so every launch shape and a graph alone or in a batch give the same bits).
+// The epilogue adds gravity + attraction, writes the total force, and forms the group's
+// swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
+// ---------------------------------------------------------------------------------------
+constexpr int kWalkStack = 512;     // stack entries per wave
+constexpr int kWalkSlack = 48;      // > tree levels + 2: the depth-first rounds' headroom
+constexpr int kWalkList = 256;      // accepted records buffered per wave
+
+__device__ __forceinline__ float vote_d2(float px, float py, float cx, float cy, float epssq) {
+    const float dx = px - cx, dy = py - cy;
+    return fmaf(dx, dx, fmaf(dy, dy, epssq));
 }
 
-// Record fetch modes (none changes a result bit: same records, same order, same arithmetic):
-//   0  one load of record r per step;
-//   1  the group's lanes load G consecutive records at once (coalesced) and a step reads
-//      record r from the lane holding it by a lane shuffle while r stays in that window;
-//   2  records r and r+1 are loaded together and, when the step on r moves to r+1, the
-//      step on r+1 follows without another wait on memory.
-template <int G, int BT, int MODE>
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int G, int BT>
 __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, const int* __restrict__ brec,
                                               const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
                                               int ngroups, const int* __restrict__ node_off,
@@ -301,71 +290,162 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
                                               float2* __restrict__ f, float2* __restrict__ rep,
                                               WalkPart* __restrict__ part, float k_r, float itolsq, float epssq,
                                               const DevState* __restrict__ st) {
+    static_assert(G == 32 || G == 64, "vote group of 32 or 64 lanes");
+    constexpr int W = BT / 64;
+    constexpr int kStride = (G == 32) ? 2 : 1;     // list entries per drain step (one per half-wave for G = 32)
+    constexpr int kUnroll = 4;                     // drain steps per loop trip
+    constexpr int kPad = kStride * kUnroll;
+    __shared__ int2 stk_all[W][kWalkStack];
+    __shared__ float4 lst_all[W][kWalkList + kPad];
     if (st->err != 0) return;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int lb = xcd_block();
-    const int gidx = (lb * BT + tid) / G;
-    const int gl = (lb * BT + tid) % G;           // lane within the group
-    const bool group_ok = gidx < ngroups;
-    int2 grp = make_int2(0, 0);
-    if (group_ok) grp = groups[gidx];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: one group per wave
+    int2* stk = stk_all[wv];
+    float4* lst = lst_all[wv];
+    const int gidx = xcd_block() * W + wv;
+    if (gidx >= ngroups) return;                  // uniform per wave; no block barriers below
+    const int2 grp = groups[gidx];
     const int g = grp.y;
-    const int k = grp.x + gl;
-    const bool active = group_ok && k < node_off[g + 1];
-
-    float px = 0.0f, py = 0.0f, mi = 0.0f, kmi = 0.0f;
-    unsigned int id = 0;
-    if (active) {
-        const float4 me = rec[brec[k]];
-        px = me.x;
-        py = me.y;
-        mi = me.z;
-        kmi = k_r * mi;
-        id = perm[k];
+    const int nact = min(G, node_off[g + 1] - grp.x);
+    const int bl = (G == 32) ? (lane & 31) : lane;    // this lane's body within the group
+    const bool bact = bl < nact;
+    const int k = grp.x + min(bl, nact - 1);          // inactive lanes duplicate the last body
+    const float4 me = rec[brec[k]];
+    const float px = me.x, py = me.y, mi = me.z, kmi = k_r * me.z;
+    const unsigned int id = perm[k];
+    // the group's bounding box and its first body (inactive lanes duplicate an active body)
+    float mnx = px, mny = py, mxx = px, mxy = py;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mnx = fminf(mnx, __shfl_xor(mnx, o, 64));
+        mny = fminf(mny, __shfl_xor(mny, o, 64));
+        mxx = fmaxf(mxx, __shfl_xor(mxx, o, 64));
+        mxy = fmaxf(mxy, __shfl_xor(mxy, o, 64));
     }
+    const float b0x = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(px)));
+    const float b0y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(py)));
+    // its middle and last bodies
+    const int jm = __builtin_amdgcn_readfirstlane((nact - 1) >> 1), jl = __builtin_amdgcn_readfirstlane(nact - 1);
+    const float bmx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px), jm));
+    const float bmy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), jm));
+    const float blx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px), jl));
+    const float bly = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), jl));
+    const unsigned long long voters = __builtin_amdgcn_ballot_w64(bact && lane < G);
+    const float two_r = seg_root[g].z * 2;
+    const float base = two_r * two_r * itolsq;
+    const int r0 = rec_begin[g] + 1, rend = rec_begin[g + 1];
+    int top = 0;                  // stack size (uniform)
+    int nlist = 0;                // buffered accepted records (uniform)
+    unsigned int naccepted = 0;   // accepted records of the whole walk (uniform)
+    if (r0 < rend) {
+        if (lane == 0) stk[0] = make_int2(r0, rend);
+        top = 1;
+    }
+    wave_lds_sync();
     float ax = 0.0f, ay = 0.0f;
-    unsigned int inter = 0;
-    // Every lane of a group walks, active or not, so the group's record index stays uniform.
-    const int gbase = (G == 32) ? (lane & 32) : 0;
-    if (group_ok) {
-        const unsigned long long act = __builtin_amdgcn_ballot_w64(active);
-        const float two_r = seg_root[g].z * 2;
-        const float base = two_r * two_r * itolsq;
-        int r = rec_begin[g] + 1;
-        const int end = rec_begin[g + 1];
-        if constexpr (MODE == 1) {
-            int wb = r - G;                   // window [wb, wb + G): empty at the start
-            float4 wnd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            while (r < end) {
-                if (r - wb >= G) {            // uniform within the group
-                    wb = r;
-                    const int q = r + gl;
-                    wnd = q < end ? rec[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                }
-                const int src = gbase + (r - wb);
-                float4 c;
-                c.x = __shfl(wnd.x, src, 64);
-                c.y = __shfl(wnd.y, src, 64);
-                c.z = __shfl(wnd.z, src, 64);
-                c.w = __shfl(wnd.w, src, 64);
-                r += walk_step<G>(c, true, act, px, py, kmi, base, epssq, ax, ay, inter);
-            }
-        } else if constexpr (MODE == 2) {
-            while (r < end) {
-                const bool has1 = r + 1 < end;
-                const float4 c0 = rec[r];
-                const float4 c1 = rec[has1 ? r + 1 : r];
-                const int a0 = walk_step<G>(c0, true, act, px, py, kmi, base, epssq, ax, ay, inter);
-                const bool live1 = has1 && a0 == 1;
-                const int a1 = walk_step<G>(c1, live1, act, px, py, kmi, base, epssq, ax, ay, inter);
-                r += live1 ? 1 + a1 : a0;
-            }
-        } else {
-            while (r < end) r += walk_step<G>(rec[r], true, act, px, py, kmi, base, epssq, ax, ay, inter);
+    const int half = (G == 32) ? (lane >> 5) : 0;
+    while (top > 0) {
+        int take = top < 64 ? top : 64;
+        const int room = kWalkStack - kWalkSlack - top;
+        if (room < take) take = room > 1 ? room : 1;
+        top -= take;
+        const bool live = lane < take;
+        // lanes past `take` repeat the last entry (in bounds) and take no part below
+        const int2 ent = stk[top + min(lane, take - 1)];
+        const float4 c = rec[ent.x];
+        const unsigned int info = __float_as_uint(c.w);
+        const bool body = info == 0xffffffffu;
+        const int size = body ? 1 : max((int)(info >> kInfoLevelBits), 1);
+        const float thr = ldexpf(base, -2 * (int)(info & 31u)) + epssq;
+        const float ex = c.x < mnx ? mnx - c.x : (c.x > mxx ? c.x - mxx : 0.0f);
+        const float ey = c.y < mny ? mny - c.y : (c.y > mxy ? c.y - mxy : 0.0f);
+        const bool clear = fmaf(ex, ex, fmaf(ey, ey, epssq)) >= thr;    // every body's d2 >= thr
+        // the first, middle and last bodies' own d2: any below thr rejects
+        const bool pass3 = vote_d2(b0x, b0y, c.x, c.y, epssq) >= thr && vote_d2(bmx, bmy, c.x, c.y, epssq) >= thr &&
+                           vote_d2(blx, bly, c.x, c.y, epssq) >= thr;
+        // the remaining cells, voted on by the group's bodies one cell at a time
+        unsigned long long okm = __builtin_amdgcn_ballot_w64(live && (body || clear));
+        unsigned long long um = __builtin_amdgcn_ballot_w64(live && !body && !clear && pass3);
+        while (um) {
+            const int j = __builtin_ctzll(um);
+            um &= um - 1;
+            const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.x), j));
+            const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.y), j));
+            const float tj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr), j));
+            const unsigned long long fails =
+                __builtin_amdgcn_ballot_w64(!(vote_d2(px, py, cx, cy, epssq) >= tj)) & voters;
+            if (fails == 0ull) okm |= 1ull << j;
         }
+        const bool accept = live && ((okm >> lane) & 1ull);
+        const bool open = live && !body && !accept && size > 1;
+        const bool sib = live && ent.x + size < ent.y;
+        const unsigned long long sm = __builtin_amdgcn_ballot_w64(sib);
+        const unsigned long long om = __builtin_amdgcn_ballot_w64(open);
+        const unsigned long long am = __builtin_amdgcn_ballot_w64(accept);
+        const int ns = __popcll(sm), no = __popcll(om), na = __popcll(am);
+        const int rs = __builtin_amdgcn_mbcnt_hi((unsigned)(sm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sm, 0u));
+        const int ro = __builtin_amdgcn_mbcnt_hi((unsigned)(om >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)om, 0u));
+        const int ra = __builtin_amdgcn_mbcnt_hi((unsigned)(am >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)am, 0u));
+        if (sib) stk[top + rs] = make_int2(ent.x + size, ent.y);
+        if (open) stk[top + ns + ro] = make_int2(ent.x + 1, ent.x + size);
+        if (accept) lst[nlist + ra] = make_float4(c.x, c.y, c.z, 0.0f);
+        top += ns + no;
+        nlist += na;
+        naccepted += (unsigned int)na;
+        if (nlist > kWalkList - 64 || top == 0) {
+            // pad to whole loop trips with massless entries (their terms are exactly 0)
+            const int padded = (nlist + kPad - 1) / kPad * kPad;
+            if (nlist + lane < padded) lst[nlist + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            wave_lds_sync();
+            for (int q0 = 0; q0 < padded; q0 += kPad) {
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const float4 e = lst[q0 + u * kStride + half];
+                    const float dx = px - e.x, dy = py - e.y;
+                    const float w = kmi * e.z * __builtin_amdgcn_rcpf(fmaf(dx, dx, fmaf(dy, dy, epssq)));
+                    ax = fmaf(dx, w, ax);
+                    ay = fmaf(dy, w, ay);
+                }
+            }
+            nlist = 0;
+        }
+        wave_lds_sync();
     }
-    group_finish<G>(active, group_ok, gl, gidx, id, mi, ax, ay, inter, fga, fprev, f, rep, part);
+    if constexpr (G == 32) {
+        // even entries (lanes 0-31) + odd entries (lanes 32-63)
+        ax += __shfl_down(ax, 32, 64);
+        ay += __shfl_down(ay, 32, 64);
+    }
+    // epilogue: total force, repulsion capture, the group's swing/traction/interaction sums
+    const bool own = bact && lane < G;
+    float sw = 0.0f, tr = 0.0f;
+    unsigned int inter = own ? naccepted : 0u;
+    if (own) {
+        const float2 a = fga[id];
+        const float2 F = make_float2(a.x + ax, a.y + ay);
+        f[id] = F;
+        if (rep) rep[id] = make_float2(ax, ay);
+        const float2 P = fprev[id];
+        float dx = F.x - P.x, dy = F.y - P.y;
+        sw = mi * sqrtf(dx * dx + dy * dy);
+        dx = F.x + P.x;
+        dy = F.y + P.y;
+        tr = mi * (sqrtf(dx * dx + dy * dy) * 0.5f);
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        sw += __shfl_xor(sw, o, 64);
+        tr += __shfl_xor(tr, o, 64);
+        inter += __shfl_xor(inter, o, 64);
+    }
+    if (lane == 0) {
+        WalkPart wp;
+        wp.swing = sw;
+        wp.traction = tr;
+        wp.interactions = inter;
+        wp.pad = 0;
+        part[gidx] = wp;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -627,20 +707,13 @@ int walk_blocks(int ngroups, int group, int bt) {
 
 namespace {
 
-template <int G, int BT, int MODE>
-void walk_launch_w(const Bufs& b, const Params& p, hipStream_t s) {
-    k_walk<G, BT, MODE><<<walk_blocks(b.ngroups, G, BT), BT, 0, s>>>(
-        b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups, b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev, b.f,
-        b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
-}
-
 template <int G, int BT>
 void walk_launch(const Bufs& b, const Params& p, hipStream_t s) {
-    switch (p.walk_fetch) {
-        case 1: walk_launch_w<G, BT, 1>(b, p, s); break;
-        case 2: walk_launch_w<G, BT, 2>(b, p, s); break;
-        default: walk_launch_w<G, BT, 0>(b, p, s); break;
-    }
+    constexpr int W = BT / 64;     // one group per wave
+    const int blocks = (b.ngroups + W - 1) / W;
+    k_walk<G, BT><<<blocks < 1 ? 1 : blocks, BT, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups,
+                                                               b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev,
+                                                               b.f, b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
 }
 
 template <int G>

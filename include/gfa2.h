@@ -161,15 +161,13 @@ int gfa2_set_timing(gfa2_handle h, int32_t on);
 int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** names, double* ms);
 
 /* Launch shapes (replaces the compile-time THREADSn/FACTORn of
- * src/RPBHFA2LaunchParameters.cuh:10-59). None of them changes any result bit: the walk's
- * vote group stays 32 or 64 lanes whatever its block size, and the grid-stride kernels
+ * src/RPBHFA2LaunchParameters.cuh:10-59). None of them changes any result bit: the walk
+ * runs one vote group per wavefront whatever its block size, and the grid-stride kernels
  * only produce exact min/max partials. */
 typedef struct gfa2_launch_config {
     int32_t walk_block;         /* walk threads per block: 64, 128, 256 or 512 */
     int32_t blocks_per_cu;      /* grid of the grid-stride kernels, per compute unit */
-    int32_t walk_fetch;         /* how the walk fetches tree records: 0 one load per step, 1 a
-                                   group-wide window of consecutive records read by lane shuffles,
-                                   2 records in pairs (the second used when the walk moves on by one) */
+    int32_t walk_fetch;         /* reserved: must be 0 */
     int32_t reserved;
     double ms_per_iteration;    /* measured by gfa2_autotune */
 } gfa2_launch_config;

@@ -9,7 +9,7 @@ box size and speed state within 1e-5 relative.
 import numpy as np
 import pytest
 
-from gpugraphlayout_amd import Engine, GFA2_FORCE_ATTRACTION_GRAVITY, GFA2_FORCE_REPULSION, GFA2_FORCE_TOTAL
+from gpugraphlayout_amd import Engine, GFA2Error, GFA2_FORCE_ATTRACTION_GRAVITY, GFA2_FORCE_REPULSION, GFA2_FORCE_TOTAL
 from gpugraphlayout_amd import graphs
 
 pytestmark = pytest.mark.gpu
@@ -206,21 +206,19 @@ def test_coincident_bodies_are_finite():
     eng.close()
 
 
-@pytest.mark.parametrize("walk_block,blocks_per_cu,window,group", [
-    (64, 1, 0, 32), (128, 2, 0, 32), (512, 8, 0, 32), (256, 3, 0, 32),
-    (64, 8, 1, 32), (256, 4, 1, 32), (512, 1, 1, 32), (128, 8, 1, 64), (256, 2, 0, 64),
-    (64, 8, 2, 32), (256, 4, 2, 32), (128, 8, 2, 64), (512, 2, 2, 64)])
-def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, window, group):
-    print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu} window={window} "
-          f"group={group}", flush=True)
+@pytest.mark.parametrize("walk_block,blocks_per_cu,group", [
+    (64, 1, 32), (128, 2, 32), (512, 8, 32), (256, 3, 32), (64, 8, 64), (128, 8, 64), (256, 2, 64),
+    (512, 1, 64)])
+def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, group):
+    print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu} group={group}", flush=True)
     g, pos = make_case("rmat", 43)
     res = []
-    for shape in (None, (walk_block, blocks_per_cu, window)):
+    for shape in (None, (walk_block, blocks_per_cu)):
         eng = Engine(g.n, g.src, g.dst, scale=10.0, group_size=group)
         if shape is not None:
             eng.set_launch_config(*shape)
             assert eng.launch_config()["walk_block"] == walk_block
-            assert eng.launch_config()["walk_fetch"] == window
+            assert eng.launch_config()["blocks_per_cu"] == blocks_per_cu
         eng.set_positions(pos)
         eng.step(20)
         eng.sync()
@@ -229,6 +227,14 @@ def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, window, 
     np.testing.assert_array_equal(res[0][0].view(np.uint32), res[1][0].view(np.uint32))
     assert res[0][1]["global_speed"] == res[1][1]["global_speed"]
     assert res[0][1]["interactions_total"] == res[1][1]["interactions_total"]
+
+
+def test_reserved_launch_field_is_refused():
+    g, pos = make_case("tiny", 3)
+    eng = Engine(g.n, g.src, g.dst)
+    with pytest.raises(GFA2Error):
+        eng.set_launch_config(64, 4, 1)
+    eng.close()
 
 
 def test_autotune_restores_the_trajectory():
