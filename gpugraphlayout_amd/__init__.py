@@ -35,7 +35,7 @@ ABI_SYMBOLS = (
     "gfa2_set_graph_batch", "gfa2_get_batch_stats",
     "gfa2_autotune", "gfa2_get_launch_config", "gfa2_set_launch_config", "gfa2_tuned_config",
     "gfa2_set_edge_weights", "gfa2_set_graph_coo_weighted", "gfa2_exact_repulsion",
-    "gfa2_get_bbox", "gfa2_render",
+    "gfa2_get_bbox", "gfa2_render", "gfa2_render_noalloc",
 )
 
 
@@ -161,6 +161,7 @@ def lib() -> ctypes.CDLL:
     L.gfa2_exact_repulsion.argtypes = [_P, _I32, _P]
     L.gfa2_get_bbox.argtypes = [_P, _P]
     L.gfa2_render.argtypes = [_P, ctypes.POINTER(Raster), _P]
+    L.gfa2_render_noalloc.argtypes = [_P, ctypes.POINTER(Raster), _P]
     for name in ABI_SYMBOLS:
         fn = getattr(L, name)
         if name not in ("gfa2_default_params", "gfa2_strerror"):
@@ -352,6 +353,12 @@ class Engine:
         """The PNG picture rasterised on the device: height x width grey bytes, top row first."""
         out = np.zeros((frame.height, frame.width), np.uint8)
         _check(lib().gfa2_render(self.h, ctypes.byref(frame), _ptr(out)), "gfa2_render")
+        return out
+
+    def render_noalloc(self, frame: "Raster") -> np.ndarray:
+        """gfa2_render through the allocation-free entry point (fixed device scratch)."""
+        out = np.zeros((frame.height, frame.width), np.uint8)
+        _check(lib().gfa2_render_noalloc(self.h, ctypes.byref(frame), _ptr(out)), "gfa2_render_noalloc")
         return out
 
     def autotune(self, iters: int = 10) -> dict:

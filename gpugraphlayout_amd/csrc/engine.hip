@@ -1272,6 +1272,7 @@ const char* gfa2_strerror(int code) {
         case GFA2_ERR_NO_GRAPH: return "no graph loaded";
         case GFA2_ERR_UNSUPPORTED: return "unsupported configuration";
         case GFA2_ERR_DEVICE_WAIT: return "a bounded device wait ran out (the iteration was skipped)";
+        case GFA2_ERR_SCRATCH_LIMIT: return "request exceeds the fixed scratch of the allocation-free entry point";
         default: return "unknown error";
     }
 }

@@ -11,8 +11,28 @@
 #include "engine_internal.h"
 #include "../../include/gfa2.h"
 
+#include <mutex>
+
+// Fixed scratch of gfa2_render_noalloc (one copy per device, loaded with the code object and
+// held until the process ends): the node and edge hit counters, 2 x kNoAllocMaxPixels words,
+// then the picture, kNoAllocMaxPixels bytes; 9 MiB in all. Host code never passes this name
+// to a kernel or a copy: it uses the device address hipGetSymbolAddress gives for it.
+namespace gfa2 {
+constexpr long long kNoAllocMaxPixels = 1ll << 20;
+constexpr size_t kScratchWords = (size_t)(2 * kNoAllocMaxPixels + kNoAllocMaxPixels / 4);
+}  // namespace gfa2
+static __device__ __align__(256) unsigned int scratch[gfa2::kScratchWords];
+
 namespace gfa2 {
 namespace {
+
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;                               // also serialises the calls of one process
+unsigned int* g_scratch[kScratchMaxDevices] = {};      // device address of `scratch`, per device
+
+__global__ __launch_bounds__(256) void k_fill_zero(unsigned int* __restrict__ p, long long n) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = 0u;
+}
 
 struct Frame {
     float x0, y0, sx, sy;
@@ -189,6 +209,44 @@ int gfa2_render(gfa2_handle h, const gfa2_raster* r, uint8_t* gray) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(cnt);
     (void)hipFree(img);
+    return e == hipSuccess ? GFA2_OK : GFA2_ERR_HIP;
+}
+
+int gfa2_render_noalloc(gfa2_handle h, const gfa2_raster* r, uint8_t* gray) {
+    const Bufs* b = nullptr;
+    int device = 0;
+    hipStream_t s = nullptr;
+    if (!h || !r || !gray || r->width <= 0 || r->height <= 0) return GFA2_ERR_INVALID_ARG;
+    const long long npix = (long long)r->width * r->height;
+    if (npix > kNoAllocMaxPixels) return GFA2_ERR_SCRATCH_LIMIT;
+    if (!engine_graph(h, &b, &device, &s)) return GFA2_ERR_NO_GRAPH;
+    if (b->nseg > 1) return GFA2_ERR_UNSUPPORTED;
+    if (device < 0 || device >= kScratchMaxDevices) return GFA2_ERR_UNSUPPORTED;
+    if (hipSetDevice(device) != hipSuccess) return GFA2_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        // once per device, with that device current: this device's copy of the array
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GFA2_ERR_HIP;
+        g_scratch[device] = static_cast<unsigned int*>(p);
+    }
+    unsigned int* cnt = g_scratch[device];                       // node counters [0, npix), edges [npix, 2 npix)
+    unsigned char* img = reinterpret_cast<unsigned char*>(cnt + 2 * kNoAllocMaxPixels);
+    const Frame f{r->x0, r->y0, r->sx, r->sy, r->width, r->height};
+    // the counters keep the previous call's hits: cleared by a kernel (no memset node)
+    long long zb = (2 * npix + 255) / 256;
+    if (zb > 8192) zb = 8192;
+    k_fill_zero<<<(int)zb, 256, 0, s>>>(cnt, 2 * npix);
+    k_raster_nodes<<<(b->n + 255) / 256, 256, 0, s>>>(b->pos, b->n, f, cnt);
+    long long eb = (b->nnz + 255) / 256;
+    if (eb > 8192) eb = 8192;
+    if (eb > 0) k_raster_edges<<<(int)eb, 256, 0, s>>>(b->pos, b->rowptr, b->col, b->n, b->nnz, f, cnt + npix);
+    long long pb = (npix + 255) / 256;
+    if (pb > 8192) pb = 8192;
+    k_shade<<<(int)pb, 256, 0, s>>>(cnt, cnt + npix, npix, 1.0f - r->node_alpha, 1.0f - r->edge_alpha, img);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(gray, img, (size_t)npix, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e == hipSuccess ? GFA2_OK : GFA2_ERR_HIP;
 }
 

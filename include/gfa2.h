@@ -29,6 +29,7 @@ extern "C" {
 #define GFA2_ERR_UNSUPPORTED (-8)    /* e.g. exact mode: src/graph_viewer.cpp:93-97 */
 #define GFA2_ERR_DEVICE_WAIT (-9)    /* a bounded wait of a device kernel ran out (the iteration's results
                                         are invalid and its later kernels were skipped); not expected */
+#define GFA2_ERR_SCRATCH_LIMIT (-10) /* the request exceeds the fixed scratch of an allocation-free entry point */
 
 typedef struct gfa2_engine* gfa2_handle;
 
@@ -211,6 +212,16 @@ typedef struct gfa2_raster {
 /* Exact bounding box of the device positions: {min x, max x, min y, max y}. */
 int gfa2_get_bbox(gfa2_handle h, float* out4);
 int gfa2_render(gfa2_handle h, const gfa2_raster* r, uint8_t* gray);
+
+/* gfa2_render without any device allocation, for callers that cannot allocate (tight serving
+ * loops that draw every frame, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. Same arguments and bytes as gfa2_render; its counters and staging picture live in
+ * a fixed per-device scratch array of the library (9 MiB, held by every device the library
+ * runs on until the process ends), so pictures are limited to 1,048,576 pixels
+ * (GFA2_ERR_SCRATCH_LIMIT beyond). All calls on one device share that array: calls from one
+ * process are serialised by a lock, but calls that overlap on one device from different
+ * processes race. */
+int gfa2_render_noalloc(gfa2_handle h, const gfa2_raster* r, uint8_t* gray);
 
 /* Graph capture of the whole iteration (hipGraph), replayed per iteration. */
 int gfa2_set_graph_capture(gfa2_handle h, int32_t on);
