@@ -9,22 +9,36 @@ iteration (tree + walk + attraction + gravity + speed + displacement), all input
 resident in HBM. W untimed warm-up iterations, then K timed ones, bracketed by a device
 synchronisation on both sides.
 
-Also reported: per-kernel-group times from HIP events recorded on the engine's stream
-during the timed region, the roofline of the dominant kernel (the Barnes-Hut walk, FP32
-VALU: 12 flops per body-node interaction, interactions counted by the kernel itself),
-HBM rates of the streaming groups, and the serial CPU baseline (the oracle's restatement
-of the reference's CPUForceAtlas2, approximate mode) timed on this host on a bounded
-sample of the same workload.
+Also reported:
+  roofline      the Barnes-Hut walk's (FP32 VALU: 12 flops per body-node interaction,
+                interactions counted by the kernel itself), its time from HIP events on the
+                engine's stream over the timed region, and its HBM bytes per launch measured
+                in this run by two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a child
+                run of the same workload and launch shape, each scaled by what the counter
+                reads for a copy of known size;
+  cpu_baseline  the serial CPU restatement of the reference's CPUForceAtlas2 (approximate
+                mode) timed on this host on a bounded sample of the same workload;
+  time_to_equal_stress  (default workload) the learned initialiser on held-out stochastic
+                block model graphs (50,000 nodes): plain FA2 from a random start for 1,000
+                iterations against the network's start refined by FA2 until it reaches the
+                plain run's normalized stress, wall times including features and inference;
+  quality       final normalized stress (10^6 seeded node pairs) and swing of the GPU run and,
+                for ba10k, of the CPU baseline's full 500-iteration run from the same start.
 
 Other workloads (--workload): ba10k (Barabasi-Albert n = 10,000, m = 5, strong gravity,
 k_r = 2, 500 iterations), rmat24 (R-MAT over 2^24 ids, 128,000,000 edges, seed 3, 200
 iterations) and ba4096 (4,096 Barabasi-Albert graphs laid out as one batch, 300 iterations,
-with the estimate of laying them out one after another).
+with the same graphs laid out one after another for comparison).
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -33,67 +47,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "FA2 iterations/s (1M-node R-MAT) + time-to-equal-stress, learned vs plain"
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector, packed
+FP32_UNPACKED_TFLOPS = 78.6       # without packed FP32 (this build compiles none)
 HBM_PEAK_GBS = 8000.0
-
+MODEL = os.path.join(ROOT, "gpugraphlayout_amd", "data", "mpnn_sbm50k.npy")
 
 # Per-workload defaults: (timed iterations, CPU-baseline iterations, CPU warm-up iterations).
 WORKLOAD_DEFAULTS = {"rmat20": (500, 10, 1), "ba10k": (500, 500, 0), "rmat24": (200, 1, 0), "ba4096": (300, 10, 1)}
 
 
-def _host_graph(s, t):
-    """Reference loader semantics through the C++ host layer (O(E), hash-based)."""
-    from gpugraphlayout_amd.host import HostGraph
-    hg = HostGraph.from_edges(s, t)
-    return hg.arrays()
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_workload(name):
-    from gpugraphlayout_amd import graphs
-    if name == "rmat20":
-        s, t = graphs.rmat(20, 10_000_000, seed=1)
-        cfg = {"workload": "rmat-scale20-10M-edges", "rmat_abcd": [0.57, 0.19, 0.19, 0.05], "seed": 1,
-               "gravity": "weak", "k_g": 1.0, "k_r": 10.0, "theta": 1.0}
-        strong, k_g, k_r = False, 1.0, 10.0
-    elif name == "rmat24":
-        s, t = graphs.rmat(24, 128_000_000, seed=3)
-        cfg = {"workload": "rmat-scale24-128M-edges", "rmat_abcd": [0.57, 0.19, 0.19, 0.05], "seed": 3,
-               "gravity": "weak", "k_g": 1.0, "k_r": 10.0, "theta": 1.0}
-        strong, k_g, k_r = False, 1.0, 10.0
-    elif name == "ba10k":
-        s, t = graphs.barabasi_albert(10_000, 5, 0)
-        cfg = {"workload": "barabasi-albert-10k-m5", "seed": 0, "gravity": "strong", "k_g": 1.0, "k_r": 2.0,
-               "theta": 1.0}
-        strong, k_g, k_r = True, 1.0, 2.0
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    g = _host_graph(s, t)
-    del s, t
-    side = 2.0 * np.sqrt(g.n)
-    pos = graphs.glibc_random_positions(g.n, side, side, seed=1234)
-    cfg.update({"nodes": g.n, "edges": g.m, "init": "glibc random() after srandom(1234), square side 2*sqrt(V)"})
-    return g, pos, strong, k_g, k_r, cfg
-
-
-def build_batch(count=4096, seed=2):
-    """4,096 Barabasi-Albert graphs: n = round(exp(U[ln 100, ln 20000])), m ~ U{2..8}
-    (numpy default_rng(seed)), graph k generated with BA seed seed*2^32 + k and laid out
-    from i.i.d. uniform positions in a square of side 2*sqrt(n) (default_rng(seed*2^32 + k))."""
-    from gpugraphlayout_amd import graphs
-    rng = np.random.default_rng(seed)
-    ns = np.rint(np.exp(rng.uniform(np.log(100.0), np.log(20000.0), count))).astype(np.int64)
-    ms = rng.integers(2, 9, count)
-    out = []
-    for k in range(count):
-        n, m = int(ns[k]), int(ms[k])
-        s, t = graphs.barabasi_albert(n, m, (seed << 32) + k)
-        g = _host_graph(s, t)
-        pos = graphs.uniform_positions(g.n, 2.0 * np.sqrt(g.n), (seed << 32) + k)
-        out.append((g, pos))
-    return out
-
-
-def streaming_bytes(V, E, C):
+def streaming_bytes(V, E):
     """Compulsory bytes per iteration of each streaming group (each array once)."""
     return {
         "attraction": 4 * (V + 1) + 8 * E + 8 * V + 4 * V + 8 * V,   # rowptr, col(2E), pos, mass, fga
@@ -103,7 +70,13 @@ def streaming_bytes(V, E, C):
     }
 
 
-def cpu_baseline(g, pos, strong, k_g, k_r, iters, warm=1):
+def walk_algorithmic_bytes(V, C):
+    """The walk's compulsory bytes per launch: every record (V bodies + C cells, 16 B) read once,
+    brec and perm 4 B per body, F_ga and F' read and F written (8 B each) per body."""
+    return 16 * (V + C) + 8 * V + 24 * V
+
+
+def cpu_baseline(g, pos, strong, k_g, k_r, iters, warm=1, keep=False):
     from oracle import oracle as orc
     cb = orc.CPUBaseline(g.n, g.src, g.dst, pos, strong, k_g, k_r, use_bh=True)
     if warm:
@@ -111,15 +84,14 @@ def cpu_baseline(g, pos, strong, k_g, k_r, iters, warm=1):
     t0 = time.perf_counter()
     cb.step(iters)
     dt = time.perf_counter() - t0
+    out = {"value": iters / dt, "unit": "iterations/s", "cores": 1, "kind": "port",
+           "sample": f"{iters} iterations after {warm} warm-up of the serial CPUForceAtlas2 restatement "
+                     f"(approximate mode) on the same graph and initial layout; {dt:.1f} s on 1 host core",
+           "seconds": dt, "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
+    if keep:
+        out["_final"] = (cb.positions(), float(cb.state()[4]))
     cb.close()
-    return {"value": iters / dt, "unit": "iterations/s", "cores": 1, "kind": "port",
-            "sample": f"{iters} iterations after {warm} warm-up of the serial CPUForceAtlas2 restatement "
-                      f"(approximate mode) on the same graph and initial layout; {dt:.1f} s on 1 host core",
-            "seconds": dt}
-
-
-def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    return out
 
 
 def timed_run(eng, args, steps):
@@ -147,28 +119,140 @@ def timed_run(eng, args, steps):
     return dt, groups, s0, s1, launch, tune_s
 
 
-def walk_roofline(groups, s0, s1, steps):
+def walk_roofline(groups, s0, s1, steps, V):
     inter = (s1["interactions_total"] - s0["interactions_total"]) / steps
     walk_ms = groups["walk"]
-    walk_tflops = 12.0 * inter / (walk_ms * 1e-3) / 1e12
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "walk_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    tflops = 12.0 * inter / (walk_ms * 1e-3) / 1e12
     return {
-        "bound": "mfma",
+        "bound": "valu",
         "engine": "fp32 VALU (no MFMA: the walk is not a dense contraction)",
         "kernel": "k_walk (Barnes-Hut force walk)",
-        "achieved": walk_tflops,
+        "achieved": tflops,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": walk_tflops / FP32_PEAK_TFLOPS,
-        "traffic": traffic,
+        "frac": tflops / FP32_PEAK_TFLOPS,
+        "frac_attainable": tflops / FP32_UNPACKED_TFLOPS,
+        "attainable_peak": FP32_UNPACKED_TFLOPS,
+        "attainable_note": "the build compiles no packed FP32 (v_pk_*_f32), which halves the issuable FP32 rate",
+        "traffic": None,
         "flops_per_launch": 12.0 * inter,
         "interactions_per_iteration": inter,
         "avg_launch_ms": walk_ms,
+        "algorithmic_bytes_per_launch": walk_algorithmic_bytes(V, s1["cells"]),
     }
+
+
+# ---- HBM traffic per walk launch, measured in this run ------------------------------------
+def _counter_rows(outdir):
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _per_dispatch(rows, counter, name_part):
+    """Counter value (summed over its rows) per dispatch of kernels whose name contains name_part,
+    in dispatch order."""
+    acc = {}
+    for r in rows:
+        if r.get("Counter_Name") != counter or name_part not in r.get("Kernel_Name", ""):
+            continue
+        d = int(r.get("Dispatch_Id") or r.get("Correlation_Id") or len(acc))
+        acc[d] = acc.get(d, 0.0) + float(r["Counter_Value"])
+    return [acc[k] for k in sorted(acc)]
+
+
+def _profile(counter, child_args, tag):
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        raise RuntimeError("rocprofv3 not found")
+    outdir = tempfile.mkdtemp(prefix=f"bench_pmc_{tag}_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", counter, "-d", outdir, "-o", "run",
+           "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__)] + child_args
+    r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+    rows = _counter_rows(outdir)
+    shutil.rmtree(outdir, ignore_errors=True)
+    if not rows:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} ({tag}) gave no counters (exit {r.returncode}): "
+                           f"{(r.stderr or '')[-300:]}")
+    return rows
+
+
+def measure_traffic(args, launch, steps):
+    """Walk HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over the timed
+    iterations of a child run of this workload with this launch shape, each scaled by the ratio
+    of the known bytes a 256 MiB copy moves to what the same counter reads for it."""
+    child = ["--traffic-child", "--workload", args.workload, "--group", str(args.group), "--warmup", str(args.warmup),
+             "--steps", str(steps), "--walk-block", str(launch["walk_block"]),
+             "--blocks-per-cu", str(launch["blocks_per_cu"])]
+    cal = ["--calibrate-copy", "256"]
+    copy_bytes = 256 * (1 << 20)
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        walk = _per_dispatch(_profile(counter, child, "walk"), counter, "k_walk")[-steps:]
+        copy = _per_dispatch(_profile(counter, cal, "copy"), counter, "neg")
+        if not walk or not copy:
+            raise RuntimeError(f"no {counter} rows for the walk or the copy")
+        factor = copy_bytes / (1024.0 * float(np.median(copy)))
+        out[counter] = {"kib_per_dispatch": float(np.mean(walk)), "copy_kib": float(np.median(copy)),
+                        "factor": factor, "bytes": float(np.mean(walk)) * 1024.0 * factor,
+                        "dispatches": len(walk)}
+    out["bytes_per_launch"] = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return out
+
+
+def traffic_child(args):
+    """The workload and launch shape of the parent's timed run, under a counter collection."""
+    from gpugraphlayout_amd import Engine, workloads
+    g, pos, prm, _ = workloads.single(args.workload)
+    eng = Engine(g.n, g.src, g.dst, group_size=args.group, **prm)
+    eng.set_launch_config(args.walk_block, args.blocks_per_cu)
+    eng.set_positions(pos)
+    eng.step(args.warmup + args.steps)
+    eng.sync()
+    eng.close()
+
+
+# ---- time to equal stress (the learned initialiser) ------------------------------------
+def time_to_equal_stress(ngraphs, starts, per_start):
+    from gpugraphlayout_amd import learned
+    if not os.path.exists(MODEL):
+        return {"error": f"no trained network at {os.path.relpath(MODEL, ROOT)}"}
+    m, meta = learned.load_model(MODEL)
+    seeds = list(range(5000, 5000 + ngraphs))
+    t0 = time.perf_counter()
+    res = learned.evaluate_sbm(m, meta["target_scale"], seeds, meta["kappa"], meta["feature_steps"], meta["check"],
+                               multistart=(starts, per_start) if starts > 1 else None,
+                               log=lambda s: log("learned " + s[:160]))
+    m.close()
+    single = learned.summarise(res)
+    plain_s = [r["plain_seconds"] for r in res]
+    out = {
+        "workload": "sbm-50k-32-blocks-dirichlet2 (p_in = 8/|B|, p_out = 0.5/V), weak gravity k_g = 1, k_r = 2",
+        "held_out_seeds": [seeds[0], seeds[-1]],
+        "graphs": len(seeds),
+        "plain": "uniform random start, 1,000 FA2 iterations (hipGraph replay); NS on 10^6 seeded pairs",
+        "learned": "features + inference written into the engine, then FA2 until NS <= the plain run's final NS; "
+                   "wall time of a rerun without the checks (features, inference and iterations included)",
+        "model": os.path.relpath(MODEL, ROOT),
+        "trained_on": {k: meta.get(k) for k in ("train_seeds", "train_steps", "target_iterations", "command")},
+        "target_time_ratio": 0.5,
+        "plain_seconds_median": float(np.median(plain_s)),
+        "single_start": single,
+        "seconds": time.perf_counter() - t0,
+    }
+    if starts > 1:
+        out["multi_start"] = {
+            "starts": starts, "iterations_per_start": per_start,
+            "protocol": "network proposals from different feature seeds in turn, each refined up to the per-start "
+                        "budget with stress checks, until one reaches the plain run's NS; timed as run (checks "
+                        "included)",
+            "learned": learned.summarise_multistart(res, "multistart_learned"),
+            "random_starts_same_protocol": learned.summarise_multistart(res, "multistart_random"),
+        }
+    return out
 
 
 def base_record(args, steps, dt, value, cfg):
@@ -190,17 +274,17 @@ def base_record(args, steps, dt, value, cfg):
 
 
 def run_single(args):
-    from gpugraphlayout_amd import Engine, device_info
+    from gpugraphlayout_amd import Engine, device_info, workloads
     steps = args.steps or WORKLOAD_DEFAULTS[args.workload][0]
     cpu_iters = args.cpu_iters or WORKLOAD_DEFAULTS[args.workload][1]
     cpu_warm = WORKLOAD_DEFAULTS[args.workload][2]
     log(f"building workload {args.workload}")
     t0 = time.perf_counter()
-    g, pos, strong, k_g, k_r, cfg = build_workload(args.workload)
+    g, pos, prm, cfg = workloads.single(args.workload)
     log(f"V={g.n} E={g.m} built in {time.perf_counter() - t0:.1f}s")
 
     dev = device_info(0)
-    eng = Engine(g.n, g.src, g.dst, strong_gravity=int(strong), gravity=k_g, scale=k_r, group_size=args.group)
+    eng = Engine(g.n, g.src, g.dst, group_size=args.group, **prm)
     eng.set_positions(pos)
     eng.step(args.warmup)
     eng.sync()
@@ -208,9 +292,10 @@ def run_single(args):
     dt, groups, s0, s1, launch, tune_s = timed_run(eng, args, steps)
     value = steps / dt
 
-    V, E, C = g.n, g.m, s1["cells"]
-    sb = streaming_bytes(V, E, C)
+    V, E = g.n, g.m
+    sb = streaming_bytes(V, E)
     hbm = {k: {"bytes": b, "ms": groups[k], "GB/s": b / (groups[k] * 1e-3) / 1e9} for k, b in sb.items()}
+    roof = walk_roofline(groups, s0, s1, steps, V)
 
     replay = None
     if args.graph_replay > 0:
@@ -223,27 +308,64 @@ def run_single(args):
         replay = args.graph_replay / (time.perf_counter() - t1)
         eng.set_graph_capture(False)
     final = eng.stats()
+
+    # layout quality: a fresh run of the workload's own iteration count from the same start, so
+    # the GPU's and the CPU baseline's final layouts are comparable (ba10k runs the CPU path to
+    # the end as well)
+    quality = None
+    cpu = None
+    if not args.no_quality:
+        iters = WORKLOAD_DEFAULTS[args.workload][0]
+        eng.set_positions(pos)
+        eng.reset()
+        eng.step(iters)
+        eng.sync()
+        eng.stress_setup(1024, 977, seed=0)
+        ns_gpu, _ = eng.normalized_stress()
+        quality = {"iterations": iters, "pairs": "1,024 sources x 977 targets, seed 0, hop distances by BFS",
+                   "gpu": {"normalized_stress": ns_gpu, "final_swing": eng.stats()["swing"]}}
+        if args.workload == "ba10k" and not args.no_cpu_baseline:
+            log(f"CPU baseline: {cpu_iters} iterations (full run)")
+            cpu = cpu_baseline(g, pos, prm["strong_gravity"], prm["gravity"], prm["scale"], cpu_iters, cpu_warm,
+                               keep=True)
+            cpos, cswing = cpu.pop("_final")
+            quality["cpu"] = {"normalized_stress": eng.normalized_stress(cpos)[0], "final_swing": cswing}
     eng.close()
 
-    cpu = None
-    if not args.no_cpu_baseline:
+    if cpu is None and not args.no_cpu_baseline:
         log(f"CPU baseline: {cpu_iters} iterations")
-        cpu = cpu_baseline(g, pos, strong, k_g, k_r, cpu_iters, cpu_warm)
-        cpu["cpu_model"] = _cpu_model()
-        cpu["nproc"] = os.cpu_count()
+        cpu = cpu_baseline(g, pos, prm["strong_gravity"], prm["gravity"], prm["scale"], cpu_iters, cpu_warm)
+
+    if args.traffic:
+        try:
+            t0 = time.perf_counter()
+            tr = measure_traffic(args, launch, steps)
+            roof["traffic"] = tr["bytes_per_launch"]
+            roof["traffic_counters"] = {k: tr[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}
+            log(f"walk traffic {tr['bytes_per_launch'] / 1e6:.1f} MB per launch in {time.perf_counter() - t0:.0f}s")
+        except Exception as exc:   # the line still reports everything else
+            roof["traffic_error"] = str(exc)[:400]
+            log(f"traffic not measured: {exc}")
+
+    tte = None
+    if args.workload == "rmat20" and args.learned_graphs > 0:
+        log(f"time to equal stress: {args.learned_graphs} held-out SBM graphs")
+        tte = time_to_equal_stress(args.learned_graphs, args.learned_starts, args.learned_per_start)
 
     out = base_record(args, steps, dt, value,
                       dict(cfg, group_size=args.group, parallelism="single-gpu",
                            walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"]))
     out.update({
-        "roofline": walk_roofline(groups, s0, s1, steps),
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "time_to_equal_stress": tte,
+        "quality": quality,
         "nodes_edges_per_s": (V + E) * value,
         "kernel_groups_ms": groups,
         "hbm_streaming": hbm,
         "graph_replay_iterations_per_s": replay,
         "autotune_s": tune_s,
-        "cells": C,
+        "cells": s1["cells"],
         "max_level": s1["max_level"],
         "final_swing": final["swing"],
         "device": dev,
@@ -254,25 +376,27 @@ def run_single(args):
 def run_batch(args):
     """Batched layout of the 4,096-graph set: one iteration advances every graph. Compared
     with laying out a seeded sample of the graphs one after another (each alone on the
-    device, its iteration replayed as a captured hipGraph) scaled to the whole set."""
-    from gpugraphlayout_amd import BatchEngine, Engine, device_info
+    device, its iteration replayed as a captured hipGraph) scaled to the whole set; the
+    sampled graphs' final layouts are compared with the batch's bit for bit."""
+    from gpugraphlayout_amd import BatchEngine, Engine, device_info, workloads
     steps = args.steps or WORKLOAD_DEFAULTS["ba4096"][0]
-    strong, k_g, k_r = True, 1.0, 2.0
+    prm = workloads.BATCH_PARAMS
     log("building 4096 Barabasi-Albert graphs")
     t0 = time.perf_counter()
-    items = build_batch()
+    items = workloads.batch4096()
     V = sum(g.n for g, _ in items)
     E = sum(g.m for g, _ in items)
+    off = np.concatenate([[0], np.cumsum([g.n for g, _ in items])])
     log(f"sum V={V} sum E={E} built in {time.perf_counter() - t0:.1f}s")
     dev = device_info(0)
-    eng = BatchEngine([(g.n, g.src, g.dst) for g, _ in items], strong_gravity=int(strong), gravity=k_g, scale=k_r,
-                      group_size=args.group)
+    eng = BatchEngine([(g.n, g.src, g.dst) for g, _ in items], group_size=args.group, **prm)
     eng.set_positions(np.concatenate([p for _, p in items]))
     eng.step(args.warmup)
     eng.sync()
     log(f"warm-up done ({args.warmup} it), timing {steps} it")
     dt, groups, s0, s1, launch, tune_s = timed_run(eng, args, steps)
     value = steps / dt
+    batch_final = eng.positions()
     replay = None
     if args.graph_replay > 0:
         # the whole batch's iteration captured once as a hipGraph, then replayed
@@ -290,9 +414,10 @@ def run_batch(args):
     rng = np.random.default_rng(7)
     sample = np.sort(rng.choice(len(items), args.batch_sample, replace=False))
     seq_s = 0.0
+    same = 0
     for k in sample:
         g, pos = items[k]
-        e1 = Engine(g.n, g.src, g.dst, strong_gravity=int(strong), gravity=k_g, scale=k_r, group_size=args.group)
+        e1 = Engine(g.n, g.src, g.dst, group_size=args.group, **prm)
         e1.set_graph_capture(True)
         e1.set_positions(pos)
         e1.step(args.warmup)
@@ -301,20 +426,20 @@ def run_batch(args):
         e1.step(steps)
         e1.sync()
         seq_s += time.perf_counter() - t1
+        same += int(np.array_equal(e1.positions().view(np.uint32), batch_final[off[k]:off[k + 1]].view(np.uint32)))
         e1.close()
     seq_total = seq_s * len(items) / len(sample)
     log(f"sequential sample: {len(sample)} graphs, {seq_s:.2f}s -> {seq_total:.1f}s for the set "
-        f"(batched {dt:.2f}s)")
+        f"(batched {dt:.2f}s); {same}/{len(sample)} bit-identical to the batch")
 
     cpu = None
     if not args.no_cpu_baseline:
-        from oracle import oracle as orc  # noqa: F401  (checked import before timing)
         csample = sample[: args.batch_cpu_sample]
         cpu_iters = args.cpu_iters or WORKLOAD_DEFAULTS["ba4096"][1]
         tsum = 0.0
         for k in csample:
             g, pos = items[k]
-            r = cpu_baseline(g, pos, strong, k_g, k_r, cpu_iters, 1)
+            r = cpu_baseline(g, pos, prm["strong_gravity"], prm["gravity"], prm["scale"], cpu_iters, 1)
             tsum += r["seconds"] / cpu_iters
         per_batch_it = tsum * len(items) / len(csample)
         cpu = {"value": 1.0 / per_batch_it, "unit": "iterations/s", "cores": 1, "kind": "port",
@@ -324,18 +449,20 @@ def run_batch(args):
                "seconds": tsum * cpu_iters, "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
 
     cfg = {"workload": "barabasi-albert-4096-graphs", "graphs": len(items), "n": "round(exp(U[ln100, ln20000]))",
-           "m": "U{2..8}", "seed": 2, "nodes": V, "edges": E, "gravity": "strong", "k_g": k_g, "k_r": k_r,
-           "theta": 1.0, "init": "uniform square of side 2*sqrt(n) per graph", "group_size": args.group,
-           "parallelism": "single-gpu", "walk_block": launch["walk_block"], "blocks_per_cu": launch["blocks_per_cu"]}
+           "m": "U{2..8}", "seed": 2, "nodes": V, "edges": E, "gravity": "strong", "k_g": prm["gravity"],
+           "k_r": prm["scale"], "theta": 1.0, "init": "uniform square of side 2*sqrt(n) per graph",
+           "group_size": args.group, "parallelism": "single-gpu", "walk_block": launch["walk_block"],
+           "blocks_per_cu": launch["blocks_per_cu"]}
     out = base_record(args, steps, dt, value, cfg)
     out.update({
-        "roofline": walk_roofline(groups, s0, s1, steps),
+        "roofline": walk_roofline(groups, s0, s1, steps, V),
         "cpu_baseline": cpu,
         "graph_iterations_per_s": value * len(items),
         "nodes_edges_per_s": (V + E) * value,
         "kernel_groups_ms": groups,
         "sequential_estimate_s": seq_total,
         "sequential_sample_graphs": int(len(sample)),
+        "sequential_sample_bit_identical": int(same),
         "batched_s": dt,
         "batched_speedup_vs_sequential": seq_total / dt,
         "graph_replay_iterations_per_s": replay,
@@ -354,17 +481,26 @@ def main():
     ap.add_argument("--group", type=int, default=32)
     ap.add_argument("--cpu-iters", type=int, default=0, help="CPU-baseline iterations (0: the workload's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-quality", action="store_true", help="skip the final-layout quality run")
     ap.add_argument("--autotune", type=int, default=30,
                     help="iterations per candidate launch shape before timing (0: default shapes)")
     ap.add_argument("--walk-block", type=int, default=0, help="fixed walk block size (skips the autotuner)")
     ap.add_argument("--blocks-per-cu", type=int, default=8, help="with --walk-block: grid-stride blocks per CU")
     ap.add_argument("--graph-replay", type=int, default=100,
                     help="extra iterations replayed as one captured hipGraph per iteration (reported apart)")
+    ap.add_argument("--traffic", type=int, default=None,
+                    help="1: measure the walk's HBM bytes with rocprofv3 counters in child runs (default: on for "
+                         "rmat20)")
+    ap.add_argument("--learned-graphs", type=int, default=20,
+                    help="held-out SBM graphs for time-to-equal-stress (rmat20 only; 0: skip)")
+    ap.add_argument("--learned-starts", type=int, default=4, help="network proposals in the multi-start protocol")
+    ap.add_argument("--learned-per-start", type=int, default=100, help="FA2 iterations per proposal")
     ap.add_argument("--batch-sample", type=int, default=64, help="graphs laid out alone for the sequential estimate")
     ap.add_argument("--batch-cpu-sample", type=int, default=16, help="graphs timed on the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only stream MIB MiB through torch.neg (4 B read + 4 B written per element), "
                          "5 times: the known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; a run on several GPUs is not part of this build.",
@@ -380,6 +516,11 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n, "launches": 5}))
         return
+    if args.traffic_child:
+        traffic_child(args)
+        return
+    if args.traffic is None:
+        args.traffic = 1 if args.workload == "rmat20" else 0
     if args.workload == "ba4096":
         run_batch(args)
     else:

@@ -179,7 +179,10 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
         // every graph fits one block's sort: by (graph, key) in one launch
         segmented_sort_pairs(b.sort, b.node_off, b.seg_order, b.nseg, s);
     } else {
-        radix_sort_pairs(b.sort, b.n, 0, 64, s);
+        // the previous tree's depth bounds the key bits the order needs (skipped low passes are
+        // made up for by the fix-up, so the order is the full sort's whatever the depth)
+        radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st);
+        launch_sort_fixup(b.sort, b.n, b.st, s);
         if (b.nseg > 1) launch_seg_graph_sort(b, s);
     }
     if (ev) hipEventRecord(ev[3], s);
@@ -235,6 +238,12 @@ int reset_state(gfa2_engine* h) {
 }
 
 int build_groups(gfa2_engine* h);
+
+// New positions or parameters: the previous tree's depth says nothing about the next one, so
+// the next path-key sort sorts every key bit.
+hipError_t full_sort_next(gfa2_engine* h) {
+    return hipMemsetAsync(&h->b.st->sort_skip, 0, sizeof(int), h->stream);
+}
 
 // The ABI status of the device error word (tree overflow first: it is the one a caller can
 // act on, with a larger cell_factor).
@@ -346,6 +355,10 @@ int gfa2_set_params(gfa2_handle h, const gfa2_params* prm) {
     if (regroup && h->has_graph) {
         const int rc = build_groups(h);
         if (rc != GFA2_OK) return rc;
+    }
+    if (h->has_graph) {
+        GFA2_TRY(hipSetDevice(h->device));
+        GFA2_TRY(full_sort_next(h));
     }
     if (new_delta && h->b.ew) {
         GFA2_TRY(hipSetDevice(h->device));
@@ -640,6 +653,7 @@ int gfa2_set_positions(gfa2_handle h, const float* xy) {
     if (!h || !xy) return GFA2_ERR_INVALID_ARG;
     if (!h->has_graph) return GFA2_ERR_NO_GRAPH;
     GFA2_TRY(hipSetDevice(h->device));
+    GFA2_TRY(full_sort_next(h));
     GFA2_TRY(hipMemcpyAsync(h->b.pos, xy, sizeof(float2) * h->b.n, hipMemcpyHostToDevice, h->stream));
     launch_bbox_partials(h->b, h->stream);
     GFA2_TRY(hipGetLastError());
@@ -1239,6 +1253,7 @@ bool engine_view(gfa2_engine* h, EngineView* v) {
 
 int engine_positions_written(gfa2_engine* h) {
     h->has_pos = true;
+    GFA2_TRY(full_sort_next(h));
     if (h->b.nseg == 1) launch_bbox_partials(h->b, h->stream);
     GFA2_TRY(hipGetLastError());
     GFA2_TRY(hipStreamSynchronize(h->stream));

@@ -49,7 +49,7 @@ struct DevState {
     int iteration;
     int nbbox;          // number of valid bounding-box partials
     int nbig;           // heavy cells listed for the in-order summary
-    int pad1;
+    int sort_skip;      // low 8-bit digit passes the next path-key sort skips (0: a full sort)
     int level_count[kMaxLevels + 1];
     int level_offset[kMaxLevels + 2];
     int level_cursor[kMaxLevels + 1];
@@ -202,8 +202,13 @@ constexpr int kTreeItems = 2;
 constexpr int kTreeTile = kTreeThreads * kTreeItems;
 
 // Radix sort of (key, value) pairs on 64-bit keys, ascending and stable. Sorts
-// buf.keys[0]/vals[0] in place (via buf.keys[1]/vals[1]).
-void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s);
+// buf.keys[0]/vals[0] in place (via buf.keys[1]/vals[1]). With skip (path keys only, begin_bit
+// 0), the first skip->sort_skip passes (an even number) do nothing: the pairs end up sorted by
+// the key's top bits only, and launch_sort_fixup then orders each run of equal top bits by the
+// whole key (and value), which gives the full sort's order.
+void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s,
+                      const DevState* skip = nullptr);
+void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s);
 // Sorts each segment [node_off[g], node_off[g+1]) of buf.keys[0]/vals[0] in place by the full
 // 64-bit key, stable, one block per segment (taken in `order`); for batches whose graphs all
 // have at most kSegSortMax nodes.

@@ -51,6 +51,27 @@ __device__ __forceinline__ void attract_term(float2 p, float2 q, int linlog, con
     }
 }
 
+// The terms of CSR entries e, e + S, e + 2S, ... below e1, added in that order. Four entries at
+// a time: their index and position loads are all issued before the first term is added (the
+// loads past e1 read node 0 and are not added), so a row's tail costs no more round trips to
+// memory than a full group of four.
+template <bool GEN, int S>
+__device__ __forceinline__ void gather_terms(float2 p, const float2* __restrict__ pos_rank,
+                                             const int* __restrict__ col, int e, int e1, int linlog,
+                                             const float* __restrict__ ew, float& ax, float& ay) {
+    for (; e < e1; e += 4 * S) {
+        int j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = e + u * S < e1 ? col[e + u * S] : 0;
+        float2 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = pos_rank[j[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e + u * S < e1) attract_term<GEN>(p, q[u], linlog, ew, e + u * S, ax, ay);
+    }
+}
+
 // col_rank[e] = rank[col[e]] (once per graph).
 __global__ __launch_bounds__(256) void k_gather_rank(const int* __restrict__ col, const int* __restrict__ rank,
                                                      long long nnz, int* __restrict__ col_rank) {
@@ -91,16 +112,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         const float2 p = pos[v];
         float ax = 0.0f, ay = 0.0f;
         const int e1 = rowptr[v + 1];
-        int e = rowptr[v];
-        for (; e + 4 <= e1; e += 4) {
-            const int j0 = col[e], j1 = col[e + 1], j2 = col[e + 2], j3 = col[e + 3];
-            const float2 q0 = pos_rank[j0], q1 = pos_rank[j1], q2 = pos_rank[j2], q3 = pos_rank[j3];
-            attract_term<GEN>(p, q0, linlog, ew, e, ax, ay);
-            attract_term<GEN>(p, q1, linlog, ew, e + 1, ax, ay);
-            attract_term<GEN>(p, q2, linlog, ew, e + 2, ax, ay);
-            attract_term<GEN>(p, q3, linlog, ew, e + 3, ax, ay);
-        }
-        for (; e < e1; ++e) attract_term<GEN>(p, pos_rank[col[e]], linlog, ew, e, ax, ay);
+        gather_terms<GEN, 1>(p, pos_rank, col, rowptr[v], e1, linlog, ew, ax, ay);
         const float2 g = gravity(p, mass[v], strong, k_g);
         fga[v] = make_float2(g.x + ax, g.y + ay);
         return;
@@ -114,16 +126,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         const float2 p = pos[v];
         float ax = 0.0f, ay = 0.0f;
         const int e1 = rowptr[v + 1];
-        int e = rowptr[v] + lane;
-        for (; e + 192 < e1; e += 256) {
-            const int j0 = col[e], j1 = col[e + 64], j2 = col[e + 128], j3 = col[e + 192];
-            const float2 q0 = pos_rank[j0], q1 = pos_rank[j1], q2 = pos_rank[j2], q3 = pos_rank[j3];
-            attract_term<GEN>(p, q0, linlog, ew, e, ax, ay);
-            attract_term<GEN>(p, q1, linlog, ew, e + 64, ax, ay);
-            attract_term<GEN>(p, q2, linlog, ew, e + 128, ax, ay);
-            attract_term<GEN>(p, q3, linlog, ew, e + 192, ax, ay);
-        }
-        for (; e < e1; e += 64) attract_term<GEN>(p, pos_rank[col[e]], linlog, ew, e, ax, ay);
+        gather_terms<GEN, 64>(p, pos_rank, col, rowptr[v] + lane, e1, linlog, ew, ax, ay);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             ax += __shfl_xor(ax, o, 64);
@@ -142,16 +145,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
     const float2 p = pos[v];
     float ax = 0.0f, ay = 0.0f;
     const int e1 = min(ch.y + kAttrChunk, rowptr[v + 1]);
-    int e = ch.y + tid;
-    for (; e + 3 * 256 < e1; e += 4 * 256) {
-        const int j0 = col[e], j1 = col[e + 256], j2 = col[e + 512], j3 = col[e + 768];
-        const float2 q0 = pos_rank[j0], q1 = pos_rank[j1], q2 = pos_rank[j2], q3 = pos_rank[j3];
-        attract_term<GEN>(p, q0, linlog, ew, e, ax, ay);
-        attract_term<GEN>(p, q1, linlog, ew, e + 256, ax, ay);
-        attract_term<GEN>(p, q2, linlog, ew, e + 512, ax, ay);
-        attract_term<GEN>(p, q3, linlog, ew, e + 768, ax, ay);
-    }
-    for (; e < e1; e += 256) attract_term<GEN>(p, pos_rank[col[e]], linlog, ew, e, ax, ay);
+    gather_terms<GEN, 256>(p, pos_rank, col, ch.y + tid, e1, linlog, ew, ax, ay);
     red[tid] = make_float2(ax, ay);
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -536,6 +530,10 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restr
             st->interactions = ri[0];
             st->global_speed = gs;
             st->speed_efficiency = se;
+            // the next path-key sort: bodies are told apart by the top 2 (max_level + 1) key bits;
+            // two levels of headroom, then whole pairs of 8-bit passes below that are skipped
+            const int need = 2 * (st->max_level + 3);
+            st->sort_skip = need >= 64 ? 0 : 2 * ((64 - need) / 16);
         }
     }
     // per graph (one block each): no word shared by the blocks of a batch

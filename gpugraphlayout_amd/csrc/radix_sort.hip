@@ -22,9 +22,15 @@ namespace {
 
 constexpr int kWaves = kSortThreads / 64;
 
+// A pass this iteration's sort skips (see radix_sort_pairs).
+__device__ __forceinline__ bool pass_skipped(const DevState* __restrict__ st, int pass) {
+    return st != nullptr && pass < st->sort_skip;
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_upsweep(const unsigned long long* __restrict__ keys,
                                                           int n, int shift, unsigned int* __restrict__ hist,
-                                                          int tiles) {
+                                                          int tiles, const DevState* __restrict__ st, int pass) {
+    if (pass_skipped(st, pass)) return;
     __shared__ unsigned int h[256];
     const int tid = threadIdx.x;
     h[tid] = 0;
@@ -46,7 +52,9 @@ __global__ __launch_bounds__(kSortThreads) void k_upsweep(const unsigned long lo
 constexpr int kRowChunks = 16;
 
 __global__ __launch_bounds__(256) void k_rowscan(unsigned int* __restrict__ hist, int tiles,
-                                                 unsigned int* __restrict__ rowtot) {
+                                                 unsigned int* __restrict__ rowtot, const DevState* __restrict__ st,
+                                                 int pass) {
+    if (pass_skipped(st, pass)) return;
     __shared__ unsigned int sh[2][4];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -202,7 +210,9 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long lo
                                                           unsigned long long* __restrict__ kout,
                                                           unsigned int* __restrict__ vout, int n, int shift,
                                                           const unsigned int* __restrict__ hist, int tiles,
-                                                          const unsigned int* __restrict__ rowtot) {
+                                                          const unsigned int* __restrict__ rowtot,
+                                                          const DevState* __restrict__ st, int pass) {
+    if (pass_skipped(st, pass)) return;
     __shared__ TileLds L;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -326,7 +336,9 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
                                                            const unsigned int* __restrict__ digit_hist,
                                                            unsigned int* __restrict__ status,
                                                            unsigned long long* __restrict__ ticket, int tiles,
-                                                           int* __restrict__ err) {
+                                                           int* __restrict__ err, const DevState* __restrict__ st,
+                                                           int pass) {
+    if (pass_skipped(st, pass)) return;   // takes no ticket: the epochs of later passes stay distinct
     __shared__ TileLds L;
     __shared__ unsigned int tcount[256];
     __shared__ int s_tile;
@@ -389,14 +401,91 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long l
     scatter_tile(key, val, tile, n, shift, gstart + excl, L, kout, vout);
 }
 
+// (key, value) pairs in lexicographic order (values are distinct body ids: a total order).
+__device__ __forceinline__ bool pair_less(unsigned long long ka, unsigned int va, unsigned long long kb,
+                                          unsigned int vb) {
+    return ka < kb || (ka == kb && va < vb);
+}
+
+// Heapsort of pairs [0, m) of k/v in place: O(m log m) whatever the input order.
+__device__ void heapsort_pairs(unsigned long long* __restrict__ k, unsigned int* __restrict__ v, int m) {
+    auto sift = [&](int root, int end) {
+        while (2 * root + 1 < end) {
+            int c = 2 * root + 1;
+            if (c + 1 < end && pair_less(k[c], v[c], k[c + 1], v[c + 1])) ++c;
+            if (!pair_less(k[root], v[root], k[c], v[c])) return;
+            const unsigned long long tk = k[root];
+            const unsigned int tv = v[root];
+            k[root] = k[c];
+            v[root] = v[c];
+            k[c] = tk;
+            v[c] = tv;
+            root = c;
+        }
+    };
+    for (int r = m / 2 - 1; r >= 0; --r) sift(r, m);
+    for (int end = m - 1; end > 0; --end) {
+        const unsigned long long tk = k[0];
+        const unsigned int tv = v[0];
+        k[0] = k[end];
+        v[0] = v[end];
+        k[end] = tk;
+        v[end] = tv;
+        sift(0, end);
+    }
+}
+
+// After a sort that skipped its low passes: every run of pairs whose keys agree in the sorted
+// top bits is ordered by (whole key, value) in place by the thread that owns the run's first
+// position (insertion sort up to 32 pairs, heapsort beyond). Such runs exist only where bodies
+// share more quadrant levels than the skip allowed for (the deepest cell grew since the previous
+// iteration, or bodies coincide), so they are few and short.
+__global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restrict__ keys,
+                                                    unsigned int* __restrict__ vals, int n,
+                                                    const DevState* __restrict__ st) {
+    const int skip = st->sort_skip;
+    if (skip == 0) return;
+    const unsigned long long top = ~0ull << (8 * skip);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i + 1 < n; i += gridDim.x * 256) {
+        const unsigned long long ki = keys[i] & top;
+        if ((keys[i + 1] & top) != ki) continue;
+        if (i > 0 && (keys[i - 1] & top) == ki) continue;     // not the first of its run
+        int j = i + 2;
+        while (j < n && (keys[j] & top) == ki) ++j;
+        if (j - i > 32) {
+            heapsort_pairs(keys + i, vals + i, j - i);
+            continue;
+        }
+        for (int a = i + 1; a < j; ++a) {
+            const unsigned long long k = keys[a];
+            const unsigned int v = vals[a];
+            int b = a - 1;
+            while (b >= i && pair_less(k, v, keys[b], vals[b])) {
+                keys[b + 1] = keys[b];
+                vals[b + 1] = vals[b];
+                --b;
+            }
+            keys[b + 1] = k;
+            vals[b + 1] = v;
+        }
+    }
+}
+
 }  // namespace
+
+void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s) {
+    if (n <= 1) return;
+    int blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], n, st);
+}
 
 void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s) {
     if (nseg > 0)
         k_seg_sort<<<nseg, kSortThreads, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], node_off, order);
 }
 
-void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s) {
+void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s, const DevState* skip) {
     if (n <= 1) return;
     const int tiles = (n + kSortTile - 1) / kSortTile;
     int cur = 0;
@@ -412,15 +501,16 @@ void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipSt
             k_onesweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], buf.vals[cur], buf.keys[cur ^ 1],
                                                       buf.vals[cur ^ 1], n, begin_bit + 8 * p,
                                                       buf.digit_hist + 256 * p, buf.status, buf.ticket, tiles,
-                                                      buf.err);
+                                                      buf.err, skip, p);
             cur ^= 1;
         }
     } else {
-        for (int shift = begin_bit; shift < end_bit; shift += 8) {
-            k_upsweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], n, shift, buf.hist, tiles);
-            k_rowscan<<<256, 256, 0, s>>>(buf.hist, tiles, buf.rowtot);
+        for (int shift = begin_bit, p = 0; shift < end_bit; shift += 8, ++p) {
+            k_upsweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], n, shift, buf.hist, tiles, skip, p);
+            k_rowscan<<<256, 256, 0, s>>>(buf.hist, tiles, buf.rowtot, skip, p);
             k_scatter<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], buf.vals[cur], buf.keys[cur ^ 1],
-                                                     buf.vals[cur ^ 1], n, shift, buf.hist, tiles, buf.rowtot);
+                                                     buf.vals[cur ^ 1], n, shift, buf.hist, tiles, buf.rowtot, skip,
+                                                     p);
             cur ^= 1;
         }
     }

@@ -153,6 +153,7 @@ struct CPUFA2 {
     float k_g = 1, k_r = 1, theta = 1.0f;
     float global_speed = 1.0f, speed_efficiency = 1.0f, jitter_tolerance = 1.0f;
     float k_s = 0.1f, k_s_max = 10.0f;
+    float last_swing = 0.0f, last_traction = 0.0f;   // S and T of the last updateSpeeds
     BH bh;
 
     float mass(nid_t v) { return (float)(g.degree(v) + 1.0); }  // src/RPForceAtlas2.cpp:74-77
@@ -234,6 +235,8 @@ struct CPUFA2 {
             S += mass(v) * swg(v);
             T += mass(v) * tra(v);
         }
+        last_swing = S;
+        last_traction = T;
         float eojt = (float)(0.05 * std::sqrt((double)g.n));
         float minjt = std::sqrt(eojt);
         float jt = jitter_tolerance * fmaxf(minjt, fminf(k_s_max, eojt * T / powf((float)g.n, 2.0f)));
@@ -315,11 +318,13 @@ void orc_cpu_get_positions(void* h, float* pos) {
     for (nid_t i = 0; i < c->g.n; ++i) { pos[2 * i] = c->pos[i].x; pos[2 * i + 1] = c->pos[i].y; }
 }
 
-// state = {global_speed, speed_efficiency, jitter_tolerance, k_s_max}
+// state = {global_speed, speed_efficiency, jitter_tolerance, k_s_max, swing, traction}
+// (swing and traction: S and T of the last iteration)
 void orc_cpu_get_state(void* h, float* state) {
     CPUFA2* c = (CPUFA2*)h;
     state[0] = c->global_speed; state[1] = c->speed_efficiency;
     state[2] = c->jitter_tolerance; state[3] = c->k_s_max;
+    state[4] = c->last_swing; state[5] = c->last_traction;
 }
 
 void orc_cpu_destroy(void* h) { delete (CPUFA2*)h; }

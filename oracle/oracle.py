@@ -244,7 +244,8 @@ class CPUBaseline:
         return out
 
     def state(self):
-        out = np.zeros(4, np.float32)
+        """{global speed, speed efficiency, jitter tolerance, k_s_max, S, T of the last iteration}."""
+        out = np.zeros(6, np.float32)
         lib().orc_cpu_get_state(self.h, _p(out))
         return out
 

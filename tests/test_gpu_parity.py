@@ -327,3 +327,66 @@ def test_attraction_depends_on_positions_only(kind):
     np.testing.assert_array_equal(fa.view(np.uint32), fb.view(np.uint32))
     a.close()
     b.close()
+
+
+def expected_order(p):
+    """The engine's body order restated with numpy: 32 quadrant digits per body from the
+    reference's float32 centre arithmetic (test_oracle.path_digits), packed two bits per level
+    from the top, sorted by key and then by node id. Also the keys."""
+    from test_oracle import path_digits
+    minx, maxx = p[:, 0].min(), p[:, 0].max()
+    miny, maxy = p[:, 1].min(), p[:, 1].max()
+    radius = np.float32(max(np.float32(maxx - minx), np.float32(maxy - miny)) * np.float32(0.5))
+    rootx, rooty = np.float32((minx + maxx) * np.float32(0.5)), np.float32((miny + maxy) * np.float32(0.5))
+    D = path_digits(p, rootx, rooty, radius, levels=32).astype(np.uint64)
+    key = np.zeros(len(p), np.uint64)
+    for lvl in range(32):
+        key |= D[:, lvl] << np.uint64(62 - 2 * lvl)
+    return np.lexsort((np.arange(len(p)), key)), key
+
+
+def test_sort_fixup_when_the_tree_deepens(oracle):
+    """The path-key sort skips low digit passes by the previous tree's depth; when the tree
+    deepens past that headroom in one iteration (pairs of bodies joined by an edge pull
+    together), the fix-up must still give the full sort's order. Every iteration's body order
+    equals a numpy sort of the restated keys, its tree is the oracle's where no bodies
+    coincide, and at least one iteration needed the fix-up."""
+    print("case sort fix-up on a deepening tree", flush=True)
+    rng = np.random.default_rng(12)
+    npairs = 600
+    centres = rng.uniform(-300, 300, size=(npairs, 2))
+    offs = rng.normal(size=(npairs, 2))
+    offs *= rng.uniform(0.4, 0.8, size=(npairs, 1)) / np.linalg.norm(offs, axis=1, keepdims=True)
+    pos = np.concatenate([centres + offs, centres - offs]).astype(np.float32)
+    src = np.arange(npairs)
+    dst = np.arange(npairs) + npairs
+    # every collapsing pair hangs a chain of cells down to the level where it parts: room for them
+    eng = Engine(2 * npairs, src, dst, scale=1e-6, gravity=0.0, strong_gravity=1, cell_factor=40)
+    eng.set_positions(pos)
+    prev_level, exercised, checked = None, 0, 0
+    for it in range(6):
+        p = eng.positions()
+        eng.step(1)
+        eng.sync()
+        T = eng.tree()
+        st = eng.stats()
+        order, key = expected_order(p)
+        np.testing.assert_array_equal(T["perm"], order, err_msg=f"iteration {it}")
+        if prev_level is not None:
+            # the bits this iteration's sort ordered by: see the sort skip in the speed step
+            need = 2 * (prev_level + 3)
+            bits = 64 - 8 * (0 if need >= 64 else 2 * ((64 - need) // 16))
+            k = key[order]
+            shift = np.uint64(64 - bits)
+            exercised += int(bits < 64 and np.any((k[1:] >> shift) == (k[:-1] >> shift)))
+        prev_level = st["max_level"]
+        print(f"iteration {it}: max level {st['max_level']}, cells {st['cells']}", flush=True)
+        if len(np.unique(p, axis=0)) < len(p):
+            continue        # coincident bodies: the reference nudges them, this engine does not
+        checked += 1
+        ref = oracle.tree(p, np.full(2 * npairs, 2.0, np.float32))
+        np.testing.assert_array_equal(T["count"], ref["count"], err_msg=f"iteration {it}")
+        np.testing.assert_array_equal(T["mass"].view(np.uint32), ref["mass"].view(np.uint32))
+    print(f"fix-up needed in {exercised} iterations, oracle trees compared in {checked}", flush=True)
+    assert exercised >= 1 and checked >= 2
+    eng.close()

@@ -380,6 +380,7 @@ def test_cpu_baseline_two_node_step(oracle):
     T = (m * np.linalg.norm(F, axis=1) / 2).sum()
     ref_state = oracle.speed_update(2, S, T, [1.0, 1.0, 1.0, 10.0])
     assert st[0] == pytest.approx(float(ref_state[0]), rel=1e-5)
+    assert st[4] == pytest.approx(S, rel=1e-5) and st[5] == pytest.approx(T, rel=1e-5)
     gs = float(st[0])
     expect = p + F * (gs / (1 + np.sqrt(gs * np.linalg.norm(F, axis=1))))[:, None]
     assert np.allclose(p1, expect, rtol=1e-5, atol=1e-5)
