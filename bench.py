@@ -13,9 +13,9 @@ Also reported:
   roofline      the Barnes-Hut walk's (FP32 VALU: 12 flops per body-node interaction,
                 interactions counted by the kernel itself), its time from HIP events on the
                 engine's stream over the timed region, and its HBM bytes per launch measured
-                in this run by two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a child
-                run of the same workload and launch shape, each scaled by what the counter
-                reads for a copy of known size;
+                in this run by two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over the first
+                50 timed iterations of a child run of the same workload and launch shape, each
+                scaled by what the counter reads for a copy of known size;
   cpu_baseline  the serial CPU restatement of the reference's CPUForceAtlas2 (approximate
                 mode) timed on this host on a bounded sample of the same workload;
   time_to_equal_stress  (default workload) the learned initialiser on held-out stochastic
@@ -163,27 +163,41 @@ def _per_dispatch(rows, counter, name_part):
     return [acc[k] for k in sorted(acc)]
 
 
-def _profile(counter, child_args, tag):
+class ProfilerCrash(RuntimeError):
+    """The profiled child ended on a signal: no further GPU work in this process."""
+
+
+def _profile(counter, child_args, tag, kernel_regex):
     exe = shutil.which("rocprofv3")
     if not exe:
         raise RuntimeError("rocprofv3 not found")
     outdir = tempfile.mkdtemp(prefix=f"bench_pmc_{tag}_", dir="/tmp")
     env = dict(os.environ, TMPDIR="/tmp")
-    cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", counter, "-d", outdir, "-o", "run",
-           "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__)] + child_args
+    # counters for the one kernel only (a counter record for every dispatch of a long run is
+    # more than the profiler holds)
+    cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", counter, "--kernel-include-regex", kernel_regex,
+           "-d", outdir, "-o", "run", "--output-format", "csv", "--", sys.executable,
+           os.path.abspath(__file__)] + child_args
     r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
     rows = _counter_rows(outdir)
     shutil.rmtree(outdir, ignore_errors=True)
     if not rows:
-        raise RuntimeError(f"rocprofv3 --pmc {counter} ({tag}) gave no counters (exit {r.returncode}): "
-                           f"{(r.stderr or '')[-300:]}")
+        msg = f"rocprofv3 --pmc {counter} ({tag}) gave no counters (exit {r.returncode})"
+        if r.returncode < 0 or r.returncode >= 128:
+            raise ProfilerCrash(msg)
+        raise RuntimeError(f"{msg}: {(r.stderr or '')[-300:]}")
     return rows
 
 
+TRAFFIC_STEPS = 50   # the profiler does not survive a counter collection over hundreds of iterations
+
+
 def measure_traffic(args, launch, steps):
-    """Walk HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over the timed
-    iterations of a child run of this workload with this launch shape, each scaled by the ratio
-    of the known bytes a 256 MiB copy moves to what the same counter reads for it."""
+    """Walk HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) averaged over the
+    first TRAFFIC_STEPS timed iterations of the same workload, start and launch shape in a child
+    run, each scaled by the ratio of the known bytes a 256 MiB copy moves to what the same
+    counter reads for it."""
+    steps = min(steps, TRAFFIC_STEPS)
     child = ["--traffic-child", "--workload", args.workload, "--group", str(args.group), "--warmup", str(args.warmup),
              "--steps", str(steps), "--walk-block", str(launch["walk_block"]),
              "--blocks-per-cu", str(launch["blocks_per_cu"])]
@@ -191,8 +205,8 @@ def measure_traffic(args, launch, steps):
     copy_bytes = 256 * (1 << 20)
     out = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        walk = _per_dispatch(_profile(counter, child, "walk"), counter, "k_walk")[-steps:]
-        copy = _per_dispatch(_profile(counter, cal, "copy"), counter, "neg")
+        walk = _per_dispatch(_profile(counter, child, "walk", "k_walk"), counter, "k_walk")[-steps:]
+        copy = _per_dispatch(_profile(counter, cal, "copy", "neg"), counter, "neg")
         if not walk or not copy:
             raise RuntimeError(f"no {counter} rows for the walk or the copy")
         factor = copy_bytes / (1024.0 * float(np.median(copy)))
@@ -336,21 +350,24 @@ def run_single(args):
         log(f"CPU baseline: {cpu_iters} iterations")
         cpu = cpu_baseline(g, pos, prm["strong_gravity"], prm["gravity"], prm["scale"], cpu_iters, cpu_warm)
 
+    tte = None
+    if args.workload == "rmat20" and args.learned_graphs > 0:
+        log(f"time to equal stress: {args.learned_graphs} held-out SBM graphs")
+        tte = time_to_equal_stress(args.learned_graphs, args.learned_starts, args.learned_per_start)
+
+    # last: the counter passes run in child processes (after a child's crash nothing else runs)
     if args.traffic:
         try:
             t0 = time.perf_counter()
             tr = measure_traffic(args, launch, steps)
             roof["traffic"] = tr["bytes_per_launch"]
             roof["traffic_counters"] = {k: tr[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}
+            roof["traffic_window"] = (f"timed iterations 1-{min(steps, TRAFFIC_STEPS)} of a child run of the same "
+                                      f"workload, start and launch shape")
             log(f"walk traffic {tr['bytes_per_launch'] / 1e6:.1f} MB per launch in {time.perf_counter() - t0:.0f}s")
         except Exception as exc:   # the line still reports everything else
             roof["traffic_error"] = str(exc)[:400]
             log(f"traffic not measured: {exc}")
-
-    tte = None
-    if args.workload == "rmat20" and args.learned_graphs > 0:
-        log(f"time to equal stress: {args.learned_graphs} held-out SBM graphs")
-        tte = time_to_equal_stress(args.learned_graphs, args.learned_starts, args.learned_per_start)
 
     out = base_record(args, steps, dt, value,
                       dict(cfg, group_size=args.group, parallelism="single-gpu",

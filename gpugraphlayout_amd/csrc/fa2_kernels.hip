@@ -307,7 +307,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     const float4 me = rec[brec[k]];
     const float px = me.x, py = me.y, mi = me.z, kmi = k_r * me.z;
     const unsigned int id = perm[k];
-    // the group's bounding box and its first body (inactive lanes duplicate an active body)
+    // the group's bounding box (inactive lanes duplicate an active body) and its first body
     float mnx = px, mny = py, mxx = px, mxy = py;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -710,8 +710,8 @@ void walk_launch(const Bufs& b, const Params& p, hipStream_t s) {
     constexpr int W = BT / 64;     // one group per wave
     const int blocks = (b.ngroups + W - 1) / W;
     k_walk<G, BT><<<blocks < 1 ? 1 : blocks, BT, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups,
-                                                               b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev,
-                                                               b.f, b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
+                                                          b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev, b.f,
+                                                          b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
 }
 
 template <int G>

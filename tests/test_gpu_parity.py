@@ -390,3 +390,24 @@ def test_sort_fixup_when_the_tree_deepens(oracle):
     print(f"fix-up needed in {exercised} iterations, oracle trees compared in {checked}", flush=True)
     assert exercised >= 1 and checked >= 2
     eng.close()
+
+
+@pytest.mark.parametrize("n", [150_000, 600_000, 1_000_000])
+def test_large_sorts_match_numpy_order(n):
+    """Body orders at the sizes of the one-launch large-tile sort passes (up to 128 tiles of
+    8,192 pairs) and beyond, on the first iteration (every key bit sorted) and on later ones
+    (low digit passes skipped by the previous depth), against a numpy sort of the restated keys."""
+    print(f"case large sort n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    pos = rng.uniform(-1000, 1000, size=(n, 2)).astype(np.float32)
+    pos[: n // 10] *= np.float32(1e-3)        # a dense core: deeper cells there
+    src, dst = np.arange(0, n - 1, 2), np.arange(1, n, 2)
+    eng = Engine(n, src, dst, scale=1.0, gravity=1.0)
+    eng.set_positions(pos)
+    for it in range(4):
+        p = eng.positions()
+        eng.step(1)
+        eng.sync()
+        order, _ = expected_order(p)
+        np.testing.assert_array_equal(eng.tree()["perm"], order, err_msg=f"iteration {it}")
+    eng.close()
