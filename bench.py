@@ -70,10 +70,11 @@ def streaming_bytes(V, E):
     }
 
 
-def walk_algorithmic_bytes(V, C):
+def walk_algorithmic_bytes(V, C, split):
     """The walk's compulsory bytes per launch: every record (V bodies + C cells, 16 B) read once,
-    brec and perm 4 B per body, F_ga and F' read and F written (8 B each) per body."""
-    return 16 * (V + C) + 8 * V + 24 * V
+    brec and perm 4 B per body, and F written (8 B) per body; the fused epilogue (one stream)
+    also reads F_ga and F' (8 B each), which the split walk leaves to the combine kernel."""
+    return 16 * (V + C) + 8 * V + (8 if split else 24) * V
 
 
 def cpu_baseline(g, pos, strong, k_g, k_r, iters, warm=1, keep=False):
@@ -98,7 +99,7 @@ def timed_run(eng, args, steps):
     """Autotune (optional), then `steps` timed iterations with per-group HIP events."""
     tune_s = None
     if args.walk_block > 0:
-        eng.set_launch_config(args.walk_block, args.blocks_per_cu)
+        eng.set_launch_config(args.walk_block, args.blocks_per_cu, args.single_stream)
     launch = eng.launch_config()
     if args.autotune > 0 and args.walk_block <= 0:
         t0 = time.perf_counter()
@@ -119,7 +120,7 @@ def timed_run(eng, args, steps):
     return dt, groups, s0, s1, launch, tune_s
 
 
-def walk_roofline(groups, s0, s1, steps, V):
+def walk_roofline(groups, s0, s1, steps, V, split):
     inter = (s1["interactions_total"] - s0["interactions_total"]) / steps
     walk_ms = groups["walk"]
     tflops = 12.0 * inter / (walk_ms * 1e-3) / 1e12
@@ -138,7 +139,7 @@ def walk_roofline(groups, s0, s1, steps, V):
         "flops_per_launch": 12.0 * inter,
         "interactions_per_iteration": inter,
         "avg_launch_ms": walk_ms,
-        "algorithmic_bytes_per_launch": walk_algorithmic_bytes(V, s1["cells"]),
+        "algorithmic_bytes_per_launch": walk_algorithmic_bytes(V, s1["cells"], split),
     }
 
 
@@ -200,7 +201,8 @@ def measure_traffic(args, launch, steps):
     steps = min(steps, TRAFFIC_STEPS)
     child = ["--traffic-child", "--workload", args.workload, "--group", str(args.group), "--warmup", str(args.warmup),
              "--steps", str(steps), "--walk-block", str(launch["walk_block"]),
-             "--blocks-per-cu", str(launch["blocks_per_cu"])]
+             "--blocks-per-cu", str(launch["blocks_per_cu"]),
+             "--single-stream", str(launch["single_stream"])]
     cal = ["--calibrate-copy", "256"]
     copy_bytes = 256 * (1 << 20)
     out = {}
@@ -222,7 +224,7 @@ def traffic_child(args):
     from gpugraphlayout_amd import Engine, workloads
     g, pos, prm, _ = workloads.single(args.workload)
     eng = Engine(g.n, g.src, g.dst, group_size=args.group, **prm)
-    eng.set_launch_config(args.walk_block, args.blocks_per_cu)
+    eng.set_launch_config(args.walk_block, args.blocks_per_cu, args.single_stream)
     eng.set_positions(pos)
     eng.step(args.warmup + args.steps)
     eng.sync()
@@ -309,7 +311,7 @@ def run_single(args):
     V, E = g.n, g.m
     sb = streaming_bytes(V, E)
     hbm = {k: {"bytes": b, "ms": groups[k], "GB/s": b / (groups[k] * 1e-3) / 1e9} for k, b in sb.items()}
-    roof = walk_roofline(groups, s0, s1, steps, V)
+    roof = walk_roofline(groups, s0, s1, steps, V, not launch["single_stream"])
 
     replay = None
     if args.graph_replay > 0:
@@ -371,7 +373,8 @@ def run_single(args):
 
     out = base_record(args, steps, dt, value,
                       dict(cfg, group_size=args.group, parallelism="single-gpu",
-                           walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"]))
+                           walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"],
+                           single_stream=launch["single_stream"]))
     out.update({
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -469,10 +472,10 @@ def run_batch(args):
            "m": "U{2..8}", "seed": 2, "nodes": V, "edges": E, "gravity": "strong", "k_g": prm["gravity"],
            "k_r": prm["scale"], "theta": 1.0, "init": "uniform square of side 2*sqrt(n) per graph",
            "group_size": args.group, "parallelism": "single-gpu", "walk_block": launch["walk_block"],
-           "blocks_per_cu": launch["blocks_per_cu"]}
+           "blocks_per_cu": launch["blocks_per_cu"], "single_stream": launch["single_stream"]}
     out = base_record(args, steps, dt, value, cfg)
     out.update({
-        "roofline": walk_roofline(groups, s0, s1, steps, V),
+        "roofline": walk_roofline(groups, s0, s1, steps, V, not launch["single_stream"]),
         "cpu_baseline": cpu,
         "graph_iterations_per_s": value * len(items),
         "nodes_edges_per_s": (V + E) * value,
@@ -503,6 +506,9 @@ def main():
                     help="iterations per candidate launch shape before timing (0: default shapes)")
     ap.add_argument("--walk-block", type=int, default=0, help="fixed walk block size (skips the autotuner)")
     ap.add_argument("--blocks-per-cu", type=int, default=8, help="with --walk-block: grid-stride blocks per CU")
+    ap.add_argument("--single-stream", type=int, default=1, choices=(0, 1),
+                    help="with --walk-block: 1 runs every kernel on one stream; 0 runs the attraction on a "
+                         "second stream beside the walk")
     ap.add_argument("--graph-replay", type=int, default=100,
                     help="extra iterations replayed as one captured hipGraph per iteration (reported apart)")
     ap.add_argument("--traffic", type=int, default=None,

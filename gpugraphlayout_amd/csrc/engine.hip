@@ -18,14 +18,23 @@ using namespace gfa2;
 
 namespace {
 
-const char* const kGroupNames[] = {"attraction", "keys", "sort", "tree", "walk", "speed", "displacement"};
-constexpr int kGroups = 7;
+const char* const kGroupNames[] = {"attraction", "keys", "sort", "tree", "walk", "combine", "speed", "displacement"};
+constexpr int kGroups = 8;
+// Timing events recorded per iteration, and the pair of events that brackets each group. The
+// attraction's pair is recorded on the stream it runs on (the second stream when it runs beside
+// the walk); "combine" is the wait for that stream plus k_combine (empty on one stream).
+// Events 0 and 11 bracket the whole iteration.
+constexpr int kEvents = 12;
+constexpr int kSpan[kGroups][2] = {{1, 2}, {3, 4}, {4, 5}, {5, 6}, {7, 8}, {8, 9}, {9, 10}, {10, 11}};
 
 }  // namespace
 
 struct gfa2_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;         // the attraction's stream (default priority, like `stream`)
+    hipEvent_t fork = nullptr;          // stream -> side: the iteration's inputs are in place
+    hipEvent_t join = nullptr;          // side -> stream: the attraction + gravity is written
     gfa2_params user{};
     Params p{};
     Bufs b{};
@@ -38,7 +47,7 @@ struct gfa2_engine {
     std::vector<void*> allocs;
     // timing
     bool timing = false;
-    std::vector<hipEvent_t> events;     // pool, kGroups+1 per pending iteration
+    std::vector<hipEvent_t> events;     // pool, kEvents per pending iteration
     int pending = 0;                    // iterations with recorded, unread events
     double acc_ms[kGroups] = {0};
     long long timed_iters = 0;
@@ -59,7 +68,7 @@ struct gfa2_engine {
     int groups_cap = 0;
     // launch shapes (autotuned per graph-size bucket)
     int cus = 256;
-    LaunchConfig tune{256, 8, 0};
+    LaunchConfig tune{256, 8, 1};
 };
 
 namespace {
@@ -85,7 +94,6 @@ int validate_params(const gfa2_params& p) {
 
 void apply_tune(gfa2_engine* h) {
     h->p.walk_bt = h->tune.walk_bt;
-    h->p.walk_fetch = h->tune.walk_fetch;
     h->b.grid_cap = h->tune.blocks_per_cu * h->cus;
 }
 
@@ -150,50 +158,66 @@ constexpr int kStressBlocks = 512;
 void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
     hipStream_t s = h->stream;
     Bufs& b = h->b;
-    if (ev) hipEventRecord(ev[0], s);
-    launch_attraction(b, h->p, s);
-    if (ev) hipEventRecord(ev[1], s);
-    if (h->p.exact) {
-        // exact repulsion: no keys, sort or tree (their groups record empty intervals)
-        if (ev) {
-            hipEventRecord(ev[2], s);
-            hipEventRecord(ev[3], s);
-            hipEventRecord(ev[4], s);
+    auto mark = [&](int i, hipStream_t on) {
+        if (ev) hipEventRecord(ev[i], on);
+    };
+    // Two streams: gravity + attraction (memory-bound gathers) runs on the second stream beside
+    // the walk (FP32 VALU-bound). It reads only the positions, which nothing in the iteration
+    // writes before the displacement, and writes only F_ga, which the split walk does not read;
+    // k_combine then forms F as the fused epilogue would.
+    const bool two = !h->tune.single_stream && !h->p.exact;
+    mark(0, s);
+    if (!two) {
+        mark(1, s);
+        launch_attraction(b, h->p, s);
+        mark(2, s);
+    }
+    mark(3, s);
+    if (!h->p.exact) {
+        if (b.nseg > 1) {
+            launch_seg_bbox(b, s);
+            launch_seg_keys(b, s);
+        } else {
+            launch_keys(b, s);
         }
-        launch_exact(b, h->p, s);
-        if (ev) hipEventRecord(ev[5], s);
-        launch_speed(b, s);
-        if (ev) hipEventRecord(ev[6], s);
-        launch_displacement(b, s);
-        if (ev) hipEventRecord(ev[7], s);
-        return;
     }
-    if (b.nseg > 1) {
-        launch_seg_bbox(b, s);
-        launch_seg_keys(b, s);
-    } else {
-        launch_keys(b, s);
+    mark(4, s);
+    if (!h->p.exact) {
+        if (b.nseg > 1 && b.max_seg <= kSegSortMax) {
+            // every graph fits one block's sort: by (graph, key) in one launch
+            segmented_sort_pairs(b.sort, b.node_off, b.seg_order, b.nseg, s);
+        } else {
+            // the previous tree's depth bounds the key bits the order needs (skipped low passes
+            // are made up for by the fix-up, so the order is the full sort's whatever the depth)
+            radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st);
+            launch_sort_fixup(b.sort, b.n, b.st, s);
+            if (b.nseg > 1) launch_seg_graph_sort(b, s);
+        }
     }
-    if (ev) hipEventRecord(ev[2], s);
-    if (b.nseg > 1 && b.max_seg <= kSegSortMax) {
-        // every graph fits one block's sort: by (graph, key) in one launch
-        segmented_sort_pairs(b.sort, b.node_off, b.seg_order, b.nseg, s);
-    } else {
-        // the previous tree's depth bounds the key bits the order needs (skipped low passes are
-        // made up for by the fix-up, so the order is the full sort's whatever the depth)
-        radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st);
-        launch_sort_fixup(b.sort, b.n, b.st, s);
-        if (b.nseg > 1) launch_seg_graph_sort(b, s);
+    mark(5, s);
+    if (!h->p.exact) launch_tree(b, s);
+    mark(6, s);
+    if (two) {
+        hipEventRecord(h->fork, s);
+        hipStreamWaitEvent(h->side, h->fork, 0);
+        mark(1, h->side);
+        launch_attraction(b, h->p, h->side);
+        mark(2, h->side);
+        hipEventRecord(h->join, h->side);
     }
-    if (ev) hipEventRecord(ev[3], s);
-    launch_tree(b, s);
-    if (ev) hipEventRecord(ev[4], s);
-    launch_walk(b, h->p, s);
-    if (ev) hipEventRecord(ev[5], s);
+    mark(7, s);
+    if (h->p.exact) launch_exact(b, h->p, s);
+    else launch_walk(b, h->p, two, s);
+    mark(8, s);
+    if (two) {
+        hipStreamWaitEvent(s, h->join, 0);
+        launch_combine(b, h->p, s);
+    }
+    mark(9, s);
     launch_speed(b, s);
-    if (ev) hipEventRecord(ev[6], s);
+    mark(10, s);
     launch_displacement(b, s);
-    if (ev) hipEventRecord(ev[7], s);
+    mark(11, s);
 }
 
 // Reads back the events of completed timed iterations.
@@ -201,10 +225,10 @@ int drain_timing(gfa2_engine* h) {
     if (h->pending == 0) return GFA2_OK;
     GFA2_TRY(hipStreamSynchronize(h->stream));
     for (int it = 0; it < h->pending; ++it) {
-        hipEvent_t* ev = &h->events[(size_t)it * (kGroups + 1)];
+        hipEvent_t* ev = &h->events[(size_t)it * kEvents];
         for (int g = 0; g < kGroups; ++g) {
             float ms = 0.0f;
-            GFA2_TRY(hipEventElapsedTime(&ms, ev[g], ev[g + 1]));
+            GFA2_TRY(hipEventElapsedTime(&ms, ev[kSpan[g][0]], ev[kSpan[g][1]]));
             h->acc_ms[g] += ms;
         }
     }
@@ -327,7 +351,13 @@ int gfa2_create(const gfa2_params* params, int device, gfa2_handle* out) {
     h->p.linlog = prm.linlog ? 1 : 0;
     h->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->join, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (h->fork) (void)hipEventDestroy(h->fork);
+        if (h->side) (void)hipStreamDestroy(h->side);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return hip_status(e);
     }
@@ -693,14 +723,14 @@ int gfa2_step(gfa2_handle h, int32_t iters) {
                 if (rc != GFA2_OK) return rc;
             }
             const int now = std::min(left, kTimedBatch - h->pending);
-            const size_t need = (size_t)(h->pending + now) * (kGroups + 1);
+            const size_t need = (size_t)(h->pending + now) * kEvents;
             while (h->events.size() < need) {
                 hipEvent_t e;
                 GFA2_TRY(hipEventCreate(&e));
                 h->events.push_back(e);
             }
             for (int i = 0; i < now; ++i) {
-                enqueue_iteration(h, &h->events[(size_t)(h->pending) * (kGroups + 1)]);
+                enqueue_iteration(h, &h->events[(size_t)(h->pending) * kEvents]);
                 h->pending++;
             }
             left -= now;
@@ -953,7 +983,7 @@ int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     c->walk_block = h->tune.walk_bt;
     c->blocks_per_cu = h->tune.blocks_per_cu;
-    c->walk_fetch = h->tune.walk_fetch;
+    c->single_stream = h->tune.single_stream;
     c->reserved = 0;
     c->ms_per_iteration = 0.0;
     return GFA2_OK;
@@ -963,9 +993,9 @@ int gfa2_set_launch_config(gfa2_handle h, const gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     const int bt = c->walk_block;
     if ((bt != 64 && bt != 128 && bt != 256 && bt != 512) || c->blocks_per_cu < 1 || c->blocks_per_cu > 64 ||
-        c->walk_fetch != 0)
+        (c->single_stream != 0 && c->single_stream != 1))
         return GFA2_ERR_INVALID_ARG;
-    h->tune = LaunchConfig{bt, c->blocks_per_cu, c->walk_fetch};
+    h->tune = LaunchConfig{bt, c->blocks_per_cu, c->single_stream};
     apply_tune(h);
     h->graph_valid = false;
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
@@ -980,7 +1010,7 @@ int gfa2_tuned_config(int32_t nodes, gfa2_launch_config* c) {
     if (it == g_tune_table.end()) return GFA2_ERR_NO_GRAPH;
     c->walk_block = it->second.walk_bt;
     c->blocks_per_cu = it->second.blocks_per_cu;
-    c->walk_fetch = it->second.walk_fetch;
+    c->single_stream = it->second.single_stream;
     c->reserved = 0;
     c->ms_per_iteration = 0.0;
     return GFA2_OK;
@@ -1077,13 +1107,15 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
         }
     };
     // Stage 1: the walk's block size (the walk dominates); stage 2: the grid of the
-    // grid-stride kernels, with the stage-1 winner.
+    // grid-stride kernels, then one stream or two, each with the winner so far.
     const int bts[] = {64, 128, 256, 512};
     const int bpcs[] = {1, 2, 4, 8};
-    for (int bt : bts) trial(LaunchConfig{bt, keep.blocks_per_cu, 0});
+    for (int bt : bts) trial(LaunchConfig{bt, keep.blocks_per_cu, keep.single_stream});
     const LaunchConfig shape = win;
     for (int bpc : bpcs)
-        if (bpc != shape.blocks_per_cu) trial(LaunchConfig{shape.walk_bt, bpc, shape.walk_fetch});
+        if (bpc != shape.blocks_per_cu) trial(LaunchConfig{shape.walk_bt, bpc, shape.single_stream});
+    const LaunchConfig shape2 = win;
+    trial(LaunchConfig{shape2.walk_bt, shape2.blocks_per_cu, 1 - shape2.single_stream});
     // Stage 3: the three fastest shapes again, three rounds each over twice the iterations,
     // interleaved so that a drift of the clock or the layout affects them alike.
     std::stable_sort(timed.begin(), timed.end(),
@@ -1121,7 +1153,7 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     if (best) {
         best->walk_block = win.walk_bt;
         best->blocks_per_cu = win.blocks_per_cu;
-        best->walk_fetch = win.walk_fetch;
+        best->single_stream = win.single_stream;
         best->reserved = 0;
         best->ms_per_iteration = win_ms / iters;
     }
@@ -1270,6 +1302,9 @@ int gfa2_destroy(gfa2_handle h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     free_all(h);
     for (hipEvent_t e : h->events) hipEventDestroy(e);
+    if (h->fork) hipEventDestroy(h->fork);
+    if (h->join) hipEventDestroy(h->join);
+    if (h->side) hipStreamDestroy(h->side);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return GFA2_OK;

@@ -95,14 +95,14 @@ struct Params {
     int exact = 0;          // 1: exact O(V^2) repulsion instead of the tree and walk
     int linlog = 0;         // attraction log(1+d)/d per unit vector
     int walk_bt = 256;      // walk threads per block (64..512); results do not depend on it
-    int walk_fetch = 0;     // reserved (0)
 };
 
 // Launch shapes the autotuner chooses from; none of them changes any result bit.
 struct LaunchConfig {
     int walk_bt;            // 64, 128, 256, 512
     int blocks_per_cu;      // grid of the grid-stride kernels (keys, displacement, bbox)
-    int walk_fetch;         // reserved (0)
+    int single_stream;      // 1: every kernel on the instance's stream; 0: the attraction on a second
+                            // stream, concurrent with a split walk (then k_combine forms F)
 };
 
 constexpr int kAttrChunk = 4096;    // entries per block for rows over 2,048 entries
@@ -221,7 +221,10 @@ void launch_col_rank(const Bufs& b, hipStream_t s);
 void launch_bbox_partials(const Bufs& b, hipStream_t s);
 void launch_keys(const Bufs& b, hipStream_t s);
 void launch_tree(Bufs& b, hipStream_t s);              // after the sort: cells + summaries
-void launch_walk(const Bufs& b, const Params& p, hipStream_t s);
+// split: the walk writes the repulsion alone (F_ga is not read), and launch_combine, after
+// both the walk and the attraction, forms F and the swing/traction sums.
+void launch_walk(const Bufs& b, const Params& p, bool split, hipStream_t s);
+void launch_combine(const Bufs& b, const Params& p, hipStream_t s);
 void launch_speed(const Bufs& b, hipStream_t s);
 void launch_displacement(const Bufs& b, hipStream_t s);
 // Exact mode: pack {pos, mass} in node order, then all-pairs repulsion per graph with the

@@ -175,6 +175,17 @@ __global__ __launch_bounds__(256) void k_attract_heavy(const float2* __restrict_
     fga[v] = make_float2(g.x + ax, g.y + ay);
 }
 
+// A body's swing and traction terms (SpeedKernel, src/RPFA2Kernels.cu:111-161) from its total
+// force F and previous force P: m |F - P| and m |F + P| / 2. The squared norms are written as
+// explicit fmaf so that every kernel that forms them (walk, combine, exact) rounds alike.
+__device__ __forceinline__ void swing_traction(float mi, float2 F, float2 P, float& sw, float& tr) {
+    float dx = F.x - P.x, dy = F.y - P.y;
+    sw = mi * sqrtf(fmaf(dx, dx, dy * dy));
+    dx = F.x + P.x;
+    dy = F.y + P.y;
+    tr = mi * (sqrtf(fmaf(dx, dx, dy * dy)) * 0.5f);
+}
+
 // The end of a body's force evaluation in the exact kernel: total force
 // F = F_ga + F_r, optional repulsion capture, and the group's swing/traction/interaction sums
 // (SpeedKernel's per-body terms, src/RPFA2Kernels.cu:111-161), reduced within the G lanes.
@@ -190,12 +201,7 @@ __device__ __forceinline__ void group_finish(bool active, bool group_ok, int gl,
         const float2 F = make_float2(a.x + ax, a.y + ay);
         f[id] = F;
         if (rep) rep[id] = make_float2(ax, ay);
-        const float2 P = fprev[id];
-        float dx = F.x - P.x, dy = F.y - P.y;
-        sw = mi * sqrtf(dx * dx + dy * dy);
-        dx = F.x + P.x;
-        dy = F.y + P.y;
-        tr = mi * (sqrtf(dx * dx + dy * dy) * 0.5f);
+        swing_traction(mi, F, fprev[id], sw, tr);
     }
     // Reduce within the group (xor offsets below G stay inside it).
 #pragma unroll
@@ -251,9 +257,11 @@ __device__ __forceinline__ int xcd_block() {
 //             depth-first in blocks of up to 64 records (one record per round near the stack's
 //             capacity, where a depth-first walk adds at most one entry per tree level).
 //  sums       when the list is nearly full, and at the end, every body adds the list's terms
-//             k_r m_i m_n d / d2 (hardware reciprocal; the reference divides, an ulp or two per
-//             term); lanes 0-31 take the even entries and lanes 32-63 the odd ones when G = 32,
-//             added together at the end.
+//             (k_r m_n) d / d2 (k_r m_n formed once per record as it is listed; hardware
+//             reciprocal, where the reference divides) and multiplies the sum by its own mass
+//             m_i at the end (the reference's k_r d m_i m_n / d2 per term, an ulp or two apart);
+//             lanes 0-31 take the even entries and lanes 32-63 the odd ones when G = 32, added
+//             together before the multiply.
 // The accepted set is the reference's; the order of the terms is the traversal's (fixed for a
 // given tree, so every launch shape and a graph alone or in a batch give the same bits).
 // The epilogue adds gravity + attraction, writes the total force, and forms the group's
@@ -305,7 +313,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     const bool bact = bl < nact;
     const int k = grp.x + min(bl, nact - 1);          // inactive lanes duplicate the last body
     const float4 me = rec[brec[k]];
-    const float px = me.x, py = me.y, mi = me.z, kmi = k_r * me.z;
+    const float px = me.x, py = me.y, mi = me.z;
     const unsigned int id = perm[k];
     // the group's bounding box (inactive lanes duplicate an active body) and its first body
     float mnx = px, mny = py, mxx = px, mxy = py;
@@ -382,7 +390,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         const int ra = __builtin_amdgcn_mbcnt_hi((unsigned)(am >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)am, 0u));
         if (sib) stk[top + rs] = make_int2(ent.x + size, ent.y);
         if (open) stk[top + ns + ro] = make_int2(ent.x + 1, ent.x + size);
-        if (accept) lst[nlist + ra] = make_float4(c.x, c.y, c.z, 0.0f);
+        if (accept) lst[nlist + ra] = make_float4(c.x, c.y, k_r * c.z, 0.0f);
         top += ns + no;
         nlist += na;
         naccepted += (unsigned int)na;
@@ -396,7 +404,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
                 for (int u = 0; u < kUnroll; ++u) {
                     const float4 e = lst[q0 + u * kStride + half];
                     const float dx = px - e.x, dy = py - e.y;
-                    const float w = kmi * e.z * __builtin_amdgcn_rcpf(fmaf(dx, dx, fmaf(dy, dy, epssq)));
+                    const float w = e.z * __builtin_amdgcn_rcpf(fmaf(dx, dx, fmaf(dy, dy, epssq)));
                     ax = fmaf(dx, w, ax);
                     ay = fmaf(dy, w, ay);
                 }
@@ -410,8 +418,25 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         ax += __shfl_down(ax, 32, 64);
         ay += __shfl_down(ay, 32, 64);
     }
-    // epilogue: total force, repulsion capture, the group's swing/traction/interaction sums
+    ax *= mi;   // the body's own mass, taken out of every term
+    ay *= mi;
     const bool own = bact && lane < G;
+    if (!fga) {
+        // split: the repulsion alone goes to f and the group's interaction count to its
+        // partial; k_combine adds F_ga and forms the swing/traction sums once the
+        // attraction, run concurrently on another stream, is done
+        if (own) f[id] = make_float2(ax, ay);
+        if (lane == 0) {
+            WalkPart wp;
+            wp.swing = 0.0f;
+            wp.traction = 0.0f;
+            wp.interactions = naccepted * (unsigned int)nact;   // = the sum over the group's bodies
+            wp.pad = 0;
+            part[gidx] = wp;
+        }
+        return;
+    }
+    // epilogue: total force, repulsion capture, the group's swing/traction/interaction sums
     float sw = 0.0f, tr = 0.0f;
     unsigned int inter = own ? naccepted : 0u;
     if (own) {
@@ -419,12 +444,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         const float2 F = make_float2(a.x + ax, a.y + ay);
         f[id] = F;
         if (rep) rep[id] = make_float2(ax, ay);
-        const float2 P = fprev[id];
-        float dx = F.x - P.x, dy = F.y - P.y;
-        sw = mi * sqrtf(dx * dx + dy * dy);
-        dx = F.x + P.x;
-        dy = F.y + P.y;
-        tr = mi * (sqrtf(dx * dx + dy * dy) * 0.5f);
+        swing_traction(mi, F, fprev[id], sw, tr);
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
@@ -439,6 +459,54 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         wp.interactions = inter;
         wp.pad = 0;
         part[gidx] = wp;
+    }
+}
+
+// The walk's epilogue for a split walk (k_walk with fga == nullptr): F = F_ga + F_r per body,
+// the optional repulsion capture, and the group's swing/traction sums, with the arithmetic and
+// the reduction order of the fused epilogue, so both give the same bits. Groups of G = 32 take
+// half a wave each (the xor offsets of the reduction stay inside a half).
+template <int G>
+__global__ __launch_bounds__(256) void k_combine(const float4* __restrict__ rec, const int* __restrict__ brec,
+                                                 const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
+                                                 int ngroups, const int* __restrict__ node_off,
+                                                 const float2* __restrict__ fga, const float2* __restrict__ fprev,
+                                                 float2* __restrict__ f, float2* __restrict__ rep,
+                                                 WalkPart* __restrict__ part, const DevState* __restrict__ st) {
+    if (st->err != 0) return;
+    const int lane = threadIdx.x & 63;
+    const int gidx = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / G) + lane / G;
+    const int bl = lane % G;
+    const bool gok = gidx < ngroups;
+    bool own = false;
+    unsigned int id = 0;
+    float mi = 0.0f;
+    if (gok) {
+        const int2 grp = groups[gidx];
+        own = bl < node_off[grp.y + 1] - grp.x;
+        if (own) {
+            const int k = grp.x + bl;
+            id = perm[k];
+            mi = rec[brec[k]].z;
+        }
+    }
+    float sw = 0.0f, tr = 0.0f;
+    if (own) {
+        const float2 r = f[id];
+        const float2 a = fga[id];
+        const float2 F = make_float2(a.x + r.x, a.y + r.y);
+        f[id] = F;
+        if (rep) rep[id] = r;
+        swing_traction(mi, F, fprev[id], sw, tr);
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        sw += __shfl_xor(sw, o, 64);
+        tr += __shfl_xor(tr, o, 64);
+    }
+    if (gok && bl == 0) {
+        part[gidx].swing = sw;
+        part[gidx].traction = tr;
     }
 }
 
@@ -706,21 +774,22 @@ int walk_blocks(int ngroups, int group, int bt) {
 namespace {
 
 template <int G, int BT>
-void walk_launch(const Bufs& b, const Params& p, hipStream_t s) {
+void walk_launch(const Bufs& b, const Params& p, bool split, hipStream_t s) {
     constexpr int W = BT / 64;     // one group per wave
     const int blocks = (b.ngroups + W - 1) / W;
     k_walk<G, BT><<<blocks < 1 ? 1 : blocks, BT, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups,
-                                                          b.node_off, b.rec_begin, b.seg_root, b.fga, b.fprev, b.f,
-                                                          b.rep, b.grp_part, p.k_r, p.itolsq, p.epssq, b.st);
+                                                          b.node_off, b.rec_begin, b.seg_root, split ? nullptr : b.fga,
+                                                          b.fprev, b.f, split ? nullptr : b.rep, b.grp_part, p.k_r,
+                                                          p.itolsq, p.epssq, b.st);
 }
 
 template <int G>
-void walk_dispatch(const Bufs& b, const Params& p, hipStream_t s) {
+void walk_dispatch(const Bufs& b, const Params& p, bool split, hipStream_t s) {
     switch (p.walk_bt) {
-        case 64: walk_launch<G, 64>(b, p, s); break;
-        case 128: walk_launch<G, 128>(b, p, s); break;
-        case 512: walk_launch<G, 512>(b, p, s); break;
-        default: walk_launch<G, 256>(b, p, s); break;
+        case 64: walk_launch<G, 64>(b, p, split, s); break;
+        case 128: walk_launch<G, 128>(b, p, split, s); break;
+        case 512: walk_launch<G, 512>(b, p, split, s); break;
+        default: walk_launch<G, 256>(b, p, split, s); break;
     }
 }
 
@@ -751,9 +820,20 @@ void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
                                                                p.k_g, b.fga);
 }
 
-void launch_walk(const Bufs& b, const Params& p, hipStream_t s) {
-    if (p.group == 64) walk_dispatch<64>(b, p, s);
-    else walk_dispatch<32>(b, p, s);
+void launch_walk(const Bufs& b, const Params& p, bool split, hipStream_t s) {
+    if (p.group == 64) walk_dispatch<64>(b, p, split, s);
+    else walk_dispatch<32>(b, p, split, s);
+}
+
+void launch_combine(const Bufs& b, const Params& p, hipStream_t s) {
+    const int per_block = 4 * (64 / p.group);
+    const int blocks = (b.ngroups + per_block - 1) / per_block;
+    if (p.group == 64)
+        k_combine<64><<<blocks < 1 ? 1 : blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups,
+                                                              b.node_off, b.fga, b.fprev, b.f, b.rep, b.grp_part, b.st);
+    else
+        k_combine<32><<<blocks < 1 ? 1 : blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups,
+                                                              b.node_off, b.fga, b.fprev, b.f, b.rep, b.grp_part, b.st);
 }
 
 void launch_speed(const Bufs& b, hipStream_t s) {

@@ -163,12 +163,16 @@ int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** nam
 
 /* Launch shapes (replaces the compile-time THREADSn/FACTORn of
  * src/RPBHFA2LaunchParameters.cuh:10-59). None of them changes any result bit: the walk
- * runs one vote group per wavefront whatever its block size, and the grid-stride kernels
- * only produce exact min/max partials. */
+ * runs one vote group per wavefront whatever its block size, the grid-stride kernels
+ * only produce exact min/max partials, and the attraction writes nothing the walk reads,
+ * so running it beside the walk changes no input of any kernel. */
 typedef struct gfa2_launch_config {
     int32_t walk_block;         /* walk threads per block: 64, 128, 256 or 512 */
     int32_t blocks_per_cu;      /* grid of the grid-stride kernels, per compute unit */
-    int32_t walk_fetch;         /* reserved: must be 0 */
+    int32_t single_stream;      /* 0: gravity + attraction on a second stream of the instance,
+                                   concurrent with the walk (which then leaves the total force
+                                   and swing/traction sums to a short combine kernel); 1: every
+                                   kernel on one stream (the kernels' own durations) */
     int32_t reserved;
     double ms_per_iteration;    /* measured by gfa2_autotune */
 } gfa2_launch_config;

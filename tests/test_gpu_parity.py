@@ -206,19 +206,20 @@ def test_coincident_bodies_are_finite():
     eng.close()
 
 
-@pytest.mark.parametrize("walk_block,blocks_per_cu,group", [
-    (64, 1, 32), (128, 2, 32), (512, 8, 32), (256, 3, 32), (64, 8, 64), (128, 8, 64), (256, 2, 64),
-    (512, 1, 64)])
-def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, group):
-    print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu} group={group}", flush=True)
+@pytest.mark.parametrize("walk_block,blocks_per_cu,single_stream,group", [
+    (64, 1, 0, 32), (128, 2, 1, 32), (512, 8, 0, 32), (256, 3, 1, 32), (64, 8, 1, 64), (128, 8, 0, 64),
+    (256, 2, 1, 64), (512, 1, 0, 64)])
+def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, single_stream, group):
+    print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu} "
+          f"single_stream={single_stream} group={group}", flush=True)
     g, pos = make_case("rmat", 43)
     res = []
-    for shape in (None, (walk_block, blocks_per_cu)):
+    for shape in (None, (walk_block, blocks_per_cu, single_stream)):
         eng = Engine(g.n, g.src, g.dst, scale=10.0, group_size=group)
         if shape is not None:
             eng.set_launch_config(*shape)
-            assert eng.launch_config()["walk_block"] == walk_block
-            assert eng.launch_config()["blocks_per_cu"] == blocks_per_cu
+            c = eng.launch_config()
+            assert (c["walk_block"], c["blocks_per_cu"], c["single_stream"]) == shape
         eng.set_positions(pos)
         eng.step(20)
         eng.sync()
@@ -229,12 +230,41 @@ def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, group):
     assert res[0][1]["interactions_total"] == res[1][1]["interactions_total"]
 
 
-def test_reserved_launch_field_is_refused():
+def test_invalid_launch_config_is_refused():
     g, pos = make_case("tiny", 3)
     eng = Engine(g.n, g.src, g.dst)
-    with pytest.raises(GFA2Error):
-        eng.set_launch_config(64, 4, 1)
+    for bad in ((64, 4, 2), (64, 4, -1), (96, 4, 0), (64, 0, 0), (64, 65, 0)):
+        with pytest.raises(GFA2Error):
+            eng.set_launch_config(*bad)
     eng.close()
+
+
+def test_streams_in_a_captured_graph_replay_bit_for_bit():
+    """The iteration with the attraction on the second stream, eager and captured as one
+    hipGraph (the second stream forks from and joins the captured stream, replayed 15 times
+    with host synchronisations between the replays), against eager single-stream launches;
+    per-group timing works with two streams."""
+    print("case two streams in a hipGraph", flush=True)
+    g, pos = make_case("rmat", 44)
+    out = []
+    for capture, single in ((False, 1), (True, 0), (False, 0)):
+        eng = Engine(g.n, g.src, g.dst, scale=10.0)
+        eng.set_launch_config(256, 4, single)
+        eng.set_graph_capture(capture)
+        eng.set_positions(pos)
+        for _ in range(3):
+            eng.step(5)
+            eng.sync()
+        out.append(eng.positions())
+        if not capture and not single:
+            eng.set_timing(True)
+            eng.step(3)
+            eng.sync()
+            t = eng.timing()
+            assert all(v >= 0 for v in t.values()) and t["walk"] > 0 and t["attraction"] > 0
+        eng.close()
+    np.testing.assert_array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+    np.testing.assert_array_equal(out[0].view(np.uint32), out[2].view(np.uint32))
 
 
 def test_autotune_restores_the_trajectory():
