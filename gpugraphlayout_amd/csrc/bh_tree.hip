@@ -586,6 +586,31 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
         atomicMax(&st->max_level, lmax);
 }
 
+// Children of every cell record, for the walk: a cell's subtree is the record run
+// [r, r + skip); its children are r + 1 and then each next record past the previous child's
+// subtree (a body spans one record), at most four. Unused slots hold -1.
+__global__ __launch_bounds__(256) void k_kids(const float4* __restrict__ rec, const int* __restrict__ S, int n,
+                                              int4* __restrict__ kids, const DevState* __restrict__ st) {
+    if (st->err != 0) return;
+    const int nrec = n + S[n];
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < nrec; r += gridDim.x * 256) {
+        const unsigned int info = __float_as_uint(rec[r].w);
+        if (info == 0xffffffffu) continue;
+        const int end = r + (int)(info >> kInfoLevelBits);
+        int k[4] = {-1, -1, -1, -1};
+        int c = r + 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (c < end) {
+                k[q] = c;
+                const unsigned int ci = __float_as_uint(rec[c].w);
+                c += ci == 0xffffffffu ? 1 : max((int)(ci >> kInfoLevelBits), 1);
+            }
+        }
+        kids[r] = make_int4(k[0], k[1], k[2], k[3]);
+    }
+}
+
 // Bucket cells by level (order inside a level is irrelevant).
 __global__ __launch_bounds__(256) void k_levels(const int* __restrict__ cell_rec,
                                                 const unsigned char* __restrict__ cell_lvl,
@@ -765,6 +790,12 @@ void launch_tree(Bufs& b, hipStream_t s) {
     k_nodes<<<node_blocks, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, sg, n, b.S, b.rec, b.rcount, b.cell_rec,
                                                  b.cell_lvl, b.rec_begin, b.st, pm, b.seg_root, b.big_rc,
                                                  b.big_lvl);
+    {
+        // every record's info word is in place (cells: k_nodes, bodies: k_prefix)
+        int kb = (2 * n + 255) / 256;
+        if (kb > 4096) kb = 4096;
+        k_kids<<<kb, 256, 0, s>>>(b.rec, b.S, n, b.kids, b.st);
+    }
     if (b.int_mass) {
         k_big<<<1, 256, 0, s>>>(b.rec, b.big_rc, b.big_lvl, b.st);
         return;

@@ -522,6 +522,7 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.blocksum, tree_tiles);
     GFA2_ALLOC(b.S, (size_t)n + 1);
     GFA2_ALLOC(b.rec, (size_t)cap);
+    GFA2_ALLOC(b.kids, (size_t)cap);
     GFA2_ALLOC(b.brec, n);
     GFA2_ALLOC(b.sbody, n);
     GFA2_ALLOC(b.rcount, (size_t)cap);

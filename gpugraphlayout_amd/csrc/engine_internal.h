@@ -140,6 +140,7 @@ struct Bufs {
     int* blocksum;
     int* S;
     float4* rec;
+    int4* kids;             // per cell record: the record indices of its 1-4 children (-1: none)
     int* brec;
     float4* sbody;          // {x, y, mass, 0} of the body at sorted position k (node k in exact mode)
     float* ew_raw = nullptr;    // edge weights per CSR entry (null: unweighted)

@@ -241,8 +241,7 @@ __device__ __forceinline__ int xcd_block() {
 // wavefront walks one group in two parts:
 //
 //  traversal  records are tested 64 at a time, one per lane, taken from a stack of pending
-//             {record, end of its parent's subtree} entries in LDS. A body record is always
-//             accepted. A cell at level L is accepted when every active body b of the group
+//             record indices in LDS. A body record is always accepted. A cell at level L is accepted when every active body b of the group
 //             has d2(b, c) >= dq[L-1] (the reference's all-lanes vote, :661), with
 //             d2 = fma(dx, dx, fma(dy, dy, eps)) and dq[d] = (2 radius)^2 itolsq 0.25^d + eps
 //             (:576-590, formed with ldexp, exact for the power-of-4 scaling). Most cells are
@@ -251,11 +250,12 @@ __device__ __forceinline__ int xcd_block() {
 //             bound of every body's d2 (accept when it clears dq), and the first, middle or
 //             last body's own d2 below dq rejects. Only the cells left are voted on, body by
 //             body, all lanes at once (lanes = bodies, as in the reference). An accepted record
-//             is appended {x, y, m} to an LDS list; an opened cell pushes its first child
-//             {r + 1, r + skip}; every tested record pushes its next sibling {r + size, end}
-//             while it lies inside its parent. The children go on top, so the traversal runs
-//             depth-first in blocks of up to 64 records (one record per round near the stack's
-//             capacity, where a depth-first walk adds at most one entry per tree level).
+//             is appended {x, y, k_r m} to an LDS list; an opened cell pushes all its 1-4
+//             children at once (their record indices, listed per cell by k_kids when the tree is
+//             built), so a cell's children are tested in the same round. The children go on
+//             top, so the traversal runs depth-first in blocks of up to 64 records (one record
+//             per round near the stack's capacity, where a depth-first walk adds at most three
+//             entries per tree level).
 //  sums       when the list is nearly full, and at the end, every body adds the list's terms
 //             (k_r m_n) d / d2 (k_r m_n formed once per record as it is listed; hardware
 //             reciprocal, where the reference divides) and multiplies the sum by its own mass
@@ -267,9 +267,10 @@ __device__ __forceinline__ int xcd_block() {
 // The epilogue adds gravity + attraction, writes the total force, and forms the group's
 // swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
 // ---------------------------------------------------------------------------------------
-constexpr int kWalkStack = 512;     // stack entries per wave
-constexpr int kWalkSlack = 48;      // > tree levels + 2: the depth-first rounds' headroom
+constexpr int kWalkStack = 1024;    // stack entries (record indices) per wave
+constexpr int kWalkSlack = 128;     // the depth-first rounds' headroom: at most 3 per tree level
 constexpr int kWalkList = 256;      // accepted records buffered per wave
+static_assert(kWalkSlack >= 3 * (kMaxLevels + 1) + 4, "a depth-first descent must fit the headroom");
 
 __device__ __forceinline__ float vote_d2(float px, float py, float cx, float cy, float epssq) {
     const float dx = px - cx, dy = py - cy;
@@ -284,7 +285,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <int G, int BT>
-__global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, const int* __restrict__ brec,
+__global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, const int4* __restrict__ kids,
+                                              const int* __restrict__ brec,
                                               const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
                                               int ngroups, const int* __restrict__ node_off,
                                               const int* __restrict__ rec_begin, const float4* __restrict__ seg_root,
@@ -297,12 +299,12 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     constexpr int kStride = (G == 32) ? 2 : 1;     // list entries per drain step (one per half-wave for G = 32)
     constexpr int kUnroll = 4;                     // drain steps per loop trip
     constexpr int kPad = kStride * kUnroll;
-    __shared__ int2 stk_all[W][kWalkStack];
+    __shared__ int stk_all[W][kWalkStack];
     __shared__ float4 lst_all[W][kWalkList + kPad];
     if (st->err != 0) return;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: one group per wave
-    int2* stk = stk_all[wv];
+    int* stk = stk_all[wv];
     float4* lst = lst_all[wv];
     const int gidx = xcd_block() * W + wv;
     if (gidx >= ngroups) return;                  // uniform per wave; no block barriers below
@@ -335,29 +337,39 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     const unsigned long long voters = __builtin_amdgcn_ballot_w64(bact && lane < G);
     const float two_r = seg_root[g].z * 2;
     const float base = two_r * two_r * itolsq;
-    const int r0 = rec_begin[g] + 1, rend = rec_begin[g + 1];
+    // the root's children start the walk (a graph of one body has no cell records)
+    const int root = rec_begin[g];
+    const bool has_root = root + 1 < rec_begin[g + 1];
     int top = 0;                  // stack size (uniform)
     int nlist = 0;                // buffered accepted records (uniform)
     unsigned int naccepted = 0;   // accepted records of the whole walk (uniform)
-    if (r0 < rend) {
-        if (lane == 0) stk[0] = make_int2(r0, rend);
-        top = 1;
+    if (has_root) {
+        const int4 k0 = kids[root];
+        if (lane == 0) {
+            stk[0] = k0.x;
+            stk[1] = k0.y;
+            stk[2] = k0.z;
+            stk[3] = k0.w;
+        }
+        top = 1 + (k0.y >= 0) + (k0.z >= 0) + (k0.w >= 0);
     }
     wave_lds_sync();
     float ax = 0.0f, ay = 0.0f;
     const int half = (G == 32) ? (lane >> 5) : 0;
     while (top > 0) {
+        // a round takes up to 64 entries; each may push up to 4 children (net +3), so near the
+        // capacity the rounds shrink to depth-first steps, whose growth the slack bounds
         int take = top < 64 ? top : 64;
-        const int room = kWalkStack - kWalkSlack - top;
+        const int room = (kWalkStack - kWalkSlack - top) / 3;
         if (room < take) take = room > 1 ? room : 1;
         top -= take;
         const bool live = lane < take;
         // lanes past `take` repeat the last entry (in bounds) and take no part below
-        const int2 ent = stk[top + min(lane, take - 1)];
-        const float4 c = rec[ent.x];
+        const int r = stk[top + min(lane, take - 1)];
+        const float4 c = rec[r];
+        const int4 kd = kids[r];      // read with the record; used only if the cell is opened
         const unsigned int info = __float_as_uint(c.w);
         const bool body = info == 0xffffffffu;
-        const int size = body ? 1 : max((int)(info >> kInfoLevelBits), 1);
         const float thr = ldexpf(base, -2 * (int)(info & 31u)) + epssq;
         const float ex = c.x < mnx ? mnx - c.x : (c.x > mxx ? c.x - mxx : 0.0f);
         const float ey = c.y < mny ? mny - c.y : (c.y > mxy ? c.y - mxy : 0.0f);
@@ -379,19 +391,35 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
             if (fails == 0ull) okm |= 1ull << j;
         }
         const bool accept = live && ((okm >> lane) & 1ull);
-        const bool open = live && !body && !accept && size > 1;
-        const bool sib = live && ent.x + size < ent.y;
-        const unsigned long long sm = __builtin_amdgcn_ballot_w64(sib);
-        const unsigned long long om = __builtin_amdgcn_ballot_w64(open);
+        const bool open = live && !body && !accept;
+        // an opened cell pushes its 1-4 children, in four planes (every opened cell's first
+        // child, then every second child, ...)
+        const unsigned long long m0 = __builtin_amdgcn_ballot_w64(open);
+        const unsigned long long m1 = __builtin_amdgcn_ballot_w64(open && kd.y >= 0);
+        const unsigned long long m2 = __builtin_amdgcn_ballot_w64(open && kd.z >= 0);
+        const unsigned long long m3 = __builtin_amdgcn_ballot_w64(open && kd.w >= 0);
         const unsigned long long am = __builtin_amdgcn_ballot_w64(accept);
-        const int ns = __popcll(sm), no = __popcll(om), na = __popcll(am);
-        const int rs = __builtin_amdgcn_mbcnt_hi((unsigned)(sm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sm, 0u));
-        const int ro = __builtin_amdgcn_mbcnt_hi((unsigned)(om >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)om, 0u));
-        const int ra = __builtin_amdgcn_mbcnt_hi((unsigned)(am >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)am, 0u));
-        if (sib) stk[top + rs] = make_int2(ent.x + size, ent.y);
-        if (open) stk[top + ns + ro] = make_int2(ent.x + 1, ent.x + size);
-        if (accept) lst[nlist + ra] = make_float4(c.x, c.y, k_r * c.z, 0.0f);
-        top += ns + no;
+        const int p1 = top + __popcll(m0), p2 = p1 + __popcll(m1), p3 = p2 + __popcll(m2);
+        const int na = __popcll(am);
+        if (open) {
+            stk[top + __builtin_amdgcn_mbcnt_hi((unsigned)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m0, 0u))] =
+                kd.x;
+            if (kd.y >= 0)
+                stk[p1 + __builtin_amdgcn_mbcnt_hi((unsigned)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m1, 0u))] =
+                    kd.y;
+            if (kd.z >= 0)
+                stk[p2 + __builtin_amdgcn_mbcnt_hi((unsigned)(m2 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m2, 0u))] =
+                    kd.z;
+            if (kd.w >= 0)
+                stk[p3 + __builtin_amdgcn_mbcnt_hi((unsigned)(m3 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m3, 0u))] =
+                    kd.w;
+        }
+        if (accept) {
+            const int ra =
+                __builtin_amdgcn_mbcnt_hi((unsigned)(am >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)am, 0u));
+            lst[nlist + ra] = make_float4(c.x, c.y, k_r * c.z, 0.0f);
+        }
+        top = p3 + __popcll(m3);
         nlist += na;
         naccepted += (unsigned int)na;
         if (nlist > kWalkList - 64 || top == 0) {
@@ -777,7 +805,7 @@ template <int G, int BT>
 void walk_launch(const Bufs& b, const Params& p, bool split, hipStream_t s) {
     constexpr int W = BT / 64;     // one group per wave
     const int blocks = (b.ngroups + W - 1) / W;
-    k_walk<G, BT><<<blocks < 1 ? 1 : blocks, BT, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups,
+    k_walk<G, BT><<<blocks < 1 ? 1 : blocks, BT, 0, s>>>(b.rec, b.kids, b.brec, b.sort.vals[0], b.groups, b.ngroups,
                                                           b.node_off, b.rec_begin, b.seg_root, split ? nullptr : b.fga,
                                                           b.fprev, b.f, split ? nullptr : b.rep, b.grp_part, p.k_r,
                                                           p.itolsq, p.epssq, b.st);
