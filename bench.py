@@ -99,7 +99,8 @@ def timed_run(eng, args, steps):
     """Autotune (optional), then `steps` timed iterations with per-group HIP events."""
     tune_s = None
     if args.walk_block > 0:
-        eng.set_launch_config(args.walk_block, args.blocks_per_cu, args.single_stream)
+        eng.set_launch_config(args.walk_block, args.blocks_per_cu, args.attraction_stream,
+                              args.attraction_blocks_per_cu)
     launch = eng.launch_config()
     if args.autotune > 0 and args.walk_block <= 0:
         t0 = time.perf_counter()
@@ -202,7 +203,8 @@ def measure_traffic(args, launch, steps):
     child = ["--traffic-child", "--workload", args.workload, "--group", str(args.group), "--warmup", str(args.warmup),
              "--steps", str(steps), "--walk-block", str(launch["walk_block"]),
              "--blocks-per-cu", str(launch["blocks_per_cu"]),
-             "--single-stream", str(launch["single_stream"])]
+             "--attraction-stream", str(launch["attraction_stream"]),
+             "--attraction-blocks-per-cu", str(launch["attraction_blocks_per_cu"])]
     cal = ["--calibrate-copy", "256"]
     copy_bytes = 256 * (1 << 20)
     out = {}
@@ -224,7 +226,7 @@ def traffic_child(args):
     from gpugraphlayout_amd import Engine, workloads
     g, pos, prm, _ = workloads.single(args.workload)
     eng = Engine(g.n, g.src, g.dst, group_size=args.group, **prm)
-    eng.set_launch_config(args.walk_block, args.blocks_per_cu, args.single_stream)
+    eng.set_launch_config(args.walk_block, args.blocks_per_cu, args.attraction_stream, args.attraction_blocks_per_cu)
     eng.set_positions(pos)
     eng.step(args.warmup + args.steps)
     eng.sync()
@@ -311,7 +313,7 @@ def run_single(args):
     V, E = g.n, g.m
     sb = streaming_bytes(V, E)
     hbm = {k: {"bytes": b, "ms": groups[k], "GB/s": b / (groups[k] * 1e-3) / 1e9} for k, b in sb.items()}
-    roof = walk_roofline(groups, s0, s1, steps, V, not launch["single_stream"])
+    roof = walk_roofline(groups, s0, s1, steps, V, launch["attraction_stream"] == 1)
 
     replay = None
     if args.graph_replay > 0:
@@ -374,7 +376,8 @@ def run_single(args):
     out = base_record(args, steps, dt, value,
                       dict(cfg, group_size=args.group, parallelism="single-gpu",
                            walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"],
-                           single_stream=launch["single_stream"]))
+                           attraction_stream=launch["attraction_stream"],
+                           attraction_blocks_per_cu=launch["attraction_blocks_per_cu"]))
     out.update({
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -472,10 +475,11 @@ def run_batch(args):
            "m": "U{2..8}", "seed": 2, "nodes": V, "edges": E, "gravity": "strong", "k_g": prm["gravity"],
            "k_r": prm["scale"], "theta": 1.0, "init": "uniform square of side 2*sqrt(n) per graph",
            "group_size": args.group, "parallelism": "single-gpu", "walk_block": launch["walk_block"],
-           "blocks_per_cu": launch["blocks_per_cu"], "single_stream": launch["single_stream"]}
+           "blocks_per_cu": launch["blocks_per_cu"], "attraction_stream": launch["attraction_stream"],
+           "attraction_blocks_per_cu": launch["attraction_blocks_per_cu"]}
     out = base_record(args, steps, dt, value, cfg)
     out.update({
-        "roofline": walk_roofline(groups, s0, s1, steps, V, not launch["single_stream"]),
+        "roofline": walk_roofline(groups, s0, s1, steps, V, launch["attraction_stream"] == 1),
         "cpu_baseline": cpu,
         "graph_iterations_per_s": value * len(items),
         "nodes_edges_per_s": (V + E) * value,
@@ -506,9 +510,11 @@ def main():
                     help="iterations per candidate launch shape before timing (0: default shapes)")
     ap.add_argument("--walk-block", type=int, default=0, help="fixed walk block size (skips the autotuner)")
     ap.add_argument("--blocks-per-cu", type=int, default=8, help="with --walk-block: grid-stride blocks per CU")
-    ap.add_argument("--single-stream", type=int, default=1, choices=(0, 1),
-                    help="with --walk-block: 1 runs every kernel on one stream; 0 runs the attraction on a "
-                         "second stream beside the walk")
+    ap.add_argument("--attraction-stream", type=int, default=0, choices=(0, 1, 2),
+                    help="with --walk-block: 0 attraction on the engine's stream; 1 on a second stream beside "
+                         "the walk; 2 on a second stream beside keys/sort/tree")
+    ap.add_argument("--attraction-blocks-per-cu", type=int, default=0,
+                    help="with --walk-block: attraction grid per compute unit (0: one block per work item)")
     ap.add_argument("--graph-replay", type=int, default=100,
                     help="extra iterations replayed as one captured hipGraph per iteration (reported apart)")
     ap.add_argument("--traffic", type=int, default=None,

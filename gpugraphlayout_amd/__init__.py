@@ -86,8 +86,8 @@ class LaunchConfig(ctypes.Structure):
     _fields_ = [
         ("walk_block", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_int32),
-        ("single_stream", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("attraction_stream", ctypes.c_int32),
+        ("attraction_blocks_per_cu", ctypes.c_int32),
         ("ms_per_iteration", ctypes.c_double),
     ]
 
@@ -373,10 +373,14 @@ class Engine:
         _check(lib().gfa2_get_launch_config(self.h, ctypes.byref(c)), "gfa2_get_launch_config")
         return c.as_dict()
 
-    def set_launch_config(self, walk_block: int, blocks_per_cu: int, single_stream: int = 0):
-        """single_stream 0: gravity + attraction on a second stream, beside keys, sort and tree;
-        1: every kernel on one stream. No setting changes a result bit."""
-        c = LaunchConfig(int(walk_block), int(blocks_per_cu), int(single_stream), 0, 0.0)
+    def set_launch_config(self, walk_block: int, blocks_per_cu: int, attraction_stream: int = 0,
+                          attraction_blocks_per_cu: int = 0):
+        """attraction_stream 0: gravity + attraction on the instance's stream; 1: on a second
+        stream beside the walk; 2: on a second stream beside keys, sort and tree, with
+        attraction_blocks_per_cu blocks per compute unit (0: one per work item). No setting
+        changes a result bit."""
+        c = LaunchConfig(int(walk_block), int(blocks_per_cu), int(attraction_stream), int(attraction_blocks_per_cu),
+                         0.0)
         _check(lib().gfa2_set_launch_config(self.h, ctypes.byref(c)), "gfa2_set_launch_config")
 
     def set_timing(self, on: bool = True):

@@ -68,7 +68,7 @@ struct gfa2_engine {
     int groups_cap = 0;
     // launch shapes (autotuned per graph-size bucket)
     int cus = 256;
-    LaunchConfig tune{256, 8, 1};
+    LaunchConfig tune{256, 8, 0, 0};
 };
 
 namespace {
@@ -95,6 +95,7 @@ int validate_params(const gfa2_params& p) {
 void apply_tune(gfa2_engine* h) {
     h->p.walk_bt = h->tune.walk_bt;
     h->b.grid_cap = h->tune.blocks_per_cu * h->cus;
+    h->b.attr_grid = h->tune.attraction_bpc * h->cus;
 }
 
 }  // namespace
@@ -161,16 +162,27 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
     auto mark = [&](int i, hipStream_t on) {
         if (ev) hipEventRecord(ev[i], on);
     };
-    // Two streams: gravity + attraction (memory-bound gathers) runs on the second stream beside
-    // the walk (FP32 VALU-bound). It reads only the positions, which nothing in the iteration
-    // writes before the displacement, and writes only F_ga, which the split walk does not read;
-    // k_combine then forms F as the fused epilogue would.
-    const bool two = !h->tune.single_stream && !h->p.exact;
+    // Gravity + attraction reads only the positions, which nothing in the iteration writes
+    // before the displacement, and writes only F_ga, which only the walk's epilogue reads. So it
+    // can run on a second stream: beside the walk (mode 1: the walk is launched split and
+    // k_combine forms F afterwards, as the fused epilogue would), or beside keys, sort and tree
+    // (mode 2: joined before the walk).
+    const int mode = h->p.exact ? 0 : h->tune.attraction_stream;
+    auto attraction_on_side = [&]() {
+        hipEventRecord(h->fork, s);
+        hipStreamWaitEvent(h->side, h->fork, 0);
+        mark(1, h->side);
+        launch_attraction(b, h->p, h->side);
+        mark(2, h->side);
+        hipEventRecord(h->join, h->side);
+    };
     mark(0, s);
-    if (!two) {
+    if (mode == 0) {
         mark(1, s);
         launch_attraction(b, h->p, s);
         mark(2, s);
+    } else if (mode == 2) {
+        attraction_on_side();
     }
     mark(3, s);
     if (!h->p.exact) {
@@ -197,19 +209,13 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
     mark(5, s);
     if (!h->p.exact) launch_tree(b, s);
     mark(6, s);
-    if (two) {
-        hipEventRecord(h->fork, s);
-        hipStreamWaitEvent(h->side, h->fork, 0);
-        mark(1, h->side);
-        launch_attraction(b, h->p, h->side);
-        mark(2, h->side);
-        hipEventRecord(h->join, h->side);
-    }
+    if (mode == 1) attraction_on_side();
+    if (mode == 2) hipStreamWaitEvent(s, h->join, 0);
     mark(7, s);
     if (h->p.exact) launch_exact(b, h->p, s);
-    else launch_walk(b, h->p, two, s);
+    else launch_walk(b, h->p, mode == 1, s);
     mark(8, s);
-    if (two) {
+    if (mode == 1) {
         hipStreamWaitEvent(s, h->join, 0);
         launch_combine(b, h->p, s);
     }
@@ -984,8 +990,8 @@ int gfa2_get_launch_config(gfa2_handle h, gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     c->walk_block = h->tune.walk_bt;
     c->blocks_per_cu = h->tune.blocks_per_cu;
-    c->single_stream = h->tune.single_stream;
-    c->reserved = 0;
+    c->attraction_stream = h->tune.attraction_stream;
+    c->attraction_blocks_per_cu = h->tune.attraction_bpc;
     c->ms_per_iteration = 0.0;
     return GFA2_OK;
 }
@@ -994,9 +1000,10 @@ int gfa2_set_launch_config(gfa2_handle h, const gfa2_launch_config* c) {
     if (!h || !c) return GFA2_ERR_INVALID_ARG;
     const int bt = c->walk_block;
     if ((bt != 64 && bt != 128 && bt != 256 && bt != 512) || c->blocks_per_cu < 1 || c->blocks_per_cu > 64 ||
-        (c->single_stream != 0 && c->single_stream != 1))
+        c->attraction_stream < 0 || c->attraction_stream > 2 || c->attraction_blocks_per_cu < 0 ||
+        c->attraction_blocks_per_cu > 64)
         return GFA2_ERR_INVALID_ARG;
-    h->tune = LaunchConfig{bt, c->blocks_per_cu, c->single_stream};
+    h->tune = LaunchConfig{bt, c->blocks_per_cu, c->attraction_stream, c->attraction_blocks_per_cu};
     apply_tune(h);
     h->graph_valid = false;
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
@@ -1011,8 +1018,8 @@ int gfa2_tuned_config(int32_t nodes, gfa2_launch_config* c) {
     if (it == g_tune_table.end()) return GFA2_ERR_NO_GRAPH;
     c->walk_block = it->second.walk_bt;
     c->blocks_per_cu = it->second.blocks_per_cu;
-    c->single_stream = it->second.single_stream;
-    c->reserved = 0;
+    c->attraction_stream = it->second.attraction_stream;
+    c->attraction_blocks_per_cu = it->second.attraction_bpc;
     c->ms_per_iteration = 0.0;
     return GFA2_OK;
 }
@@ -1108,15 +1115,20 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
         }
     };
     // Stage 1: the walk's block size (the walk dominates); stage 2: the grid of the
-    // grid-stride kernels, then one stream or two, each with the winner so far.
+    // grid-stride kernels, then where the attraction runs (and with how many blocks when it runs
+    // beside keys, sort and tree), each with the winner so far.
     const int bts[] = {64, 128, 256, 512};
     const int bpcs[] = {1, 2, 4, 8};
-    for (int bt : bts) trial(LaunchConfig{bt, keep.blocks_per_cu, keep.single_stream});
+    for (int bt : bts) trial(LaunchConfig{bt, keep.blocks_per_cu, keep.attraction_stream, keep.attraction_bpc});
     const LaunchConfig shape = win;
     for (int bpc : bpcs)
-        if (bpc != shape.blocks_per_cu) trial(LaunchConfig{shape.walk_bt, bpc, shape.single_stream});
+        if (bpc != shape.blocks_per_cu)
+            trial(LaunchConfig{shape.walk_bt, bpc, shape.attraction_stream, shape.attraction_bpc});
     const LaunchConfig shape2 = win;
-    trial(LaunchConfig{shape2.walk_bt, shape2.blocks_per_cu, 1 - shape2.single_stream});
+    const int modes[][2] = {{0, 0}, {1, 0}, {2, 0}, {2, 2}, {2, 3}, {2, 4}};
+    for (const auto& m : modes)
+        if (m[0] != shape2.attraction_stream || m[1] != shape2.attraction_bpc)
+            trial(LaunchConfig{shape2.walk_bt, shape2.blocks_per_cu, m[0], m[1]});
     // Stage 3: the three fastest shapes again, three rounds each over twice the iterations,
     // interleaved so that a drift of the clock or the layout affects them alike.
     std::stable_sort(timed.begin(), timed.end(),
@@ -1154,8 +1166,8 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     if (best) {
         best->walk_block = win.walk_bt;
         best->blocks_per_cu = win.blocks_per_cu;
-        best->single_stream = win.single_stream;
-        best->reserved = 0;
+        best->attraction_stream = win.attraction_stream;
+        best->attraction_blocks_per_cu = win.attraction_bpc;
         best->ms_per_iteration = win_ms / iters;
     }
     return GFA2_OK;

@@ -101,8 +101,9 @@ struct Params {
 struct LaunchConfig {
     int walk_bt;            // 64, 128, 256, 512
     int blocks_per_cu;      // grid of the grid-stride kernels (keys, displacement, bbox)
-    int single_stream;      // 1: every kernel on the instance's stream; 0: the attraction on a second
-                            // stream, concurrent with a split walk (then k_combine forms F)
+    int attraction_stream;  // 0: one stream; 1: attraction on a second stream beside a split walk
+                            // (k_combine then forms F); 2: second stream beside keys, sort, tree
+    int attraction_bpc;     // attraction grid per CU (0: one block per virtual block)
 };
 
 constexpr int kAttrChunk = 4096;    // entries per block for rows over 2,048 entries
@@ -122,6 +123,7 @@ struct Bufs {
     // Heavy rows are split into chunks of kAttrChunk entries, one block each; a second kernel
     // adds each heavy row's chunk partials in chunk order.
     int nattr_chunks;
+    int attr_grid = 0;      // k_attract launch blocks (0: one per virtual block)
     int2* attr_chunks;      // {heavy index h, first entry}
     int* attr_c0;           // [nheavy + 1] first chunk of each heavy row
     float2* attr_part;      // [nattr_chunks]

@@ -91,7 +91,10 @@ __global__ __launch_bounds__(256) void k_rank_pos(const float2* __restrict__ pos
 // Degree classes: light rows (<= 32 entries) one thread each, entries four at a time;
 // medium rows (<= 2,048) one wave each; heavy rows in chunks of kAttrChunk entries, one block
 // per chunk writing its partial (thread-strided, then an LDS tree), added per row in chunk order
-// by k_attract_heavy. Neighbour positions come from the degree-ranked copy.
+// by k_attract_heavy. Neighbour positions come from the degree-ranked copy. The work is a list
+// of virtual blocks (light, medium, chunks) that the grid strides over: one launch block per
+// virtual block by default, or a smaller grid that leaves room on every compute unit for
+// kernels running beside it on another stream. Every row's sum is the same whatever the grid.
 template <bool GEN>
 __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos, const float* __restrict__ mass,
                                                  const float2* __restrict__ pos_rank,
@@ -104,58 +107,64 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
                                                  const float* __restrict__ ew) {
     __shared__ float2 red[256];
     const int tid = threadIdx.x;
-    int blk = blockIdx.x;
-    if (blk < light_blocks) {
-        const int k = blk * 256 + tid;
-        if (k >= nlight) return;
-        const int v = lists[k];
-        const float2 p = pos[v];
-        float ax = 0.0f, ay = 0.0f;
-        const int e1 = rowptr[v + 1];
-        gather_terms<GEN, 1>(p, pos_rank, col, rowptr[v], e1, linlog, ew, ax, ay);
-        const float2 g = gravity(p, mass[v], strong, k_g);
-        fga[v] = make_float2(g.x + ax, g.y + ay);
-        return;
-    }
-    blk -= light_blocks;
-    if (blk < medium_blocks) {
-        const int k = blk * 4 + (tid >> 6);
-        if (k >= nmedium) return;
-        const int lane = tid & 63;
-        const int v = lists[nlight + k];
-        const float2 p = pos[v];
-        float ax = 0.0f, ay = 0.0f;
-        const int e1 = rowptr[v + 1];
-        gather_terms<GEN, 64>(p, pos_rank, col, rowptr[v] + lane, e1, linlog, ew, ax, ay);
+    const int total = light_blocks + medium_blocks + nchunks;
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {   // uniform per block
+        int blk = vb;
+        if (blk < light_blocks) {
+            const int k = blk * 256 + tid;
+            if (k < nlight) {
+                const int v = lists[k];
+                const float2 p = pos[v];
+                float ax = 0.0f, ay = 0.0f;
+                const int e1 = rowptr[v + 1];
+                gather_terms<GEN, 1>(p, pos_rank, col, rowptr[v], e1, linlog, ew, ax, ay);
+                const float2 g = gravity(p, mass[v], strong, k_g);
+                fga[v] = make_float2(g.x + ax, g.y + ay);
+            }
+            continue;
+        }
+        blk -= light_blocks;
+        if (blk < medium_blocks) {
+            const int k = blk * 4 + (tid >> 6);
+            if (k < nmedium) {    // uniform per wave
+                const int lane = tid & 63;
+                const int v = lists[nlight + k];
+                const float2 p = pos[v];
+                float ax = 0.0f, ay = 0.0f;
+                const int e1 = rowptr[v + 1];
+                gather_terms<GEN, 64>(p, pos_rank, col, rowptr[v] + lane, e1, linlog, ew, ax, ay);
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            ax += __shfl_xor(ax, o, 64);
-            ay += __shfl_xor(ay, o, 64);
+                for (int o = 32; o > 0; o >>= 1) {
+                    ax += __shfl_xor(ax, o, 64);
+                    ay += __shfl_xor(ay, o, 64);
+                }
+                if (lane == 0) {
+                    const float2 g = gravity(p, mass[v], strong, k_g);
+                    fga[v] = make_float2(g.x + ax, g.y + ay);
+                }
+            }
+            continue;
         }
-        if (lane == 0) {
-            const float2 g = gravity(p, mass[v], strong, k_g);
-            fga[v] = make_float2(g.x + ax, g.y + ay);
-        }
-        return;
-    }
-    blk -= medium_blocks;
-    if (blk >= nchunks) return;
-    const int2 ch = chunks[blk];
-    const int v = lists[nlight + nmedium + ch.x];
-    const float2 p = pos[v];
-    float ax = 0.0f, ay = 0.0f;
-    const int e1 = min(ch.y + kAttrChunk, rowptr[v + 1]);
-    gather_terms<GEN, 256>(p, pos_rank, col, ch.y + tid, e1, linlog, ew, ax, ay);
-    red[tid] = make_float2(ax, ay);
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) {
-            float2 a = red[tid], b = red[tid + o];
-            red[tid] = make_float2(a.x + b.x, a.y + b.y);
-        }
+        blk -= medium_blocks;
+        const int2 ch = chunks[blk];
+        const int v = lists[nlight + nmedium + ch.x];
+        const float2 p = pos[v];
+        float ax = 0.0f, ay = 0.0f;
+        const int e1 = min(ch.y + kAttrChunk, rowptr[v + 1]);
+        gather_terms<GEN, 256>(p, pos_rank, col, ch.y + tid, e1, linlog, ew, ax, ay);
+        red[tid] = make_float2(ax, ay);
         __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) {
+                float2 a = red[tid], b = red[tid + o];
+                red[tid] = make_float2(a.x + b.x, a.y + b.y);
+            }
+            __syncthreads();
+        }
+        if (tid == 0) part[blk] = red[0];
+        // red[0] is written again only by thread 0, and every other entry only after the
+        // barriers above, so the next virtual block may start at once
     }
-    if (tid == 0) part[blk] = red[0];
 }
 
 // Heavy rows: gravity + the row's chunk partials in chunk order.
@@ -831,7 +840,8 @@ void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
     k_rank_pos<<<(b.n + 255) / 256, 256, 0, s>>>(b.pos, b.rank, b.n, b.pos_rank);
     const int lb = (b.nlight + 255) / 256;
     const int mb = (b.nmedium + 3) / 4;
-    const int total = lb + mb + b.nattr_chunks;
+    int total = lb + mb + b.nattr_chunks;
+    if (b.attr_grid > 0 && total > b.attr_grid) total = b.attr_grid;   // the launch grid
     if (total > 0) {
         if (p.linlog || b.ew)
             k_attract<true><<<total, 256, 0, s>>>(b.pos, b.mass, b.pos_rank, b.rowptr, b.col_rank, b.node_lists, b.nlight,

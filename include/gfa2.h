@@ -164,16 +164,18 @@ int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** nam
 /* Launch shapes (replaces the compile-time THREADSn/FACTORn of
  * src/RPBHFA2LaunchParameters.cuh:10-59). None of them changes any result bit: the walk
  * runs one vote group per wavefront whatever its block size, the grid-stride kernels
- * only produce exact min/max partials, and the attraction writes nothing the walk reads,
- * so running it beside the walk changes no input of any kernel. */
+ * only produce exact min/max partials, every attraction row is summed alike whatever its
+ * grid, and the attraction reads only positions and writes only F_ga, so running it beside
+ * other kernels changes no input of any kernel. */
 typedef struct gfa2_launch_config {
     int32_t walk_block;         /* walk threads per block: 64, 128, 256 or 512 */
     int32_t blocks_per_cu;      /* grid of the grid-stride kernels, per compute unit */
-    int32_t single_stream;      /* 0: gravity + attraction on a second stream of the instance,
-                                   concurrent with the walk (which then leaves the total force
-                                   and swing/traction sums to a short combine kernel); 1: every
-                                   kernel on one stream (the kernels' own durations) */
-    int32_t reserved;
+    int32_t attraction_stream;  /* gravity + attraction: 0 on the instance's stream, before the
+                                   other kernels; 1 on a second stream, beside the walk (which
+                                   then leaves F and the swing/traction sums to a short combine
+                                   kernel); 2 on a second stream, beside keys, sort and tree */
+    int32_t attraction_blocks_per_cu;  /* attraction grid per compute unit (0: one block per
+                                   256 light rows, 4 medium rows or one heavy-row chunk) */
     double ms_per_iteration;    /* measured by gfa2_autotune */
 } gfa2_launch_config;
 

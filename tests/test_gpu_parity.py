@@ -206,20 +206,21 @@ def test_coincident_bodies_are_finite():
     eng.close()
 
 
-@pytest.mark.parametrize("walk_block,blocks_per_cu,single_stream,group", [
-    (64, 1, 0, 32), (128, 2, 1, 32), (512, 8, 0, 32), (256, 3, 1, 32), (64, 8, 1, 64), (128, 8, 0, 64),
-    (256, 2, 1, 64), (512, 1, 0, 64)])
-def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, single_stream, group):
-    print(f"case launch shape walk_block={walk_block} blocks_per_cu={blocks_per_cu} "
-          f"single_stream={single_stream} group={group}", flush=True)
+@pytest.mark.parametrize("shape,group", [
+    ((64, 1, 0, 0), 32), ((128, 2, 1, 0), 32), ((512, 8, 2, 0), 32), ((256, 3, 2, 1), 32), ((64, 8, 1, 0), 64),
+    ((128, 8, 2, 3), 64), ((256, 2, 0, 2), 64), ((512, 1, 0, 0), 64)])
+def test_launch_shapes_do_not_change_results(shape, group):
+    """(walk block, blocks per CU, attraction stream, attraction blocks per CU) against the
+    default shape: positions, speed and interaction totals bit for bit."""
+    print(f"case launch shape {shape} group={group}", flush=True)
     g, pos = make_case("rmat", 43)
     res = []
-    for shape in (None, (walk_block, blocks_per_cu, single_stream)):
+    for sh in (None, shape):
         eng = Engine(g.n, g.src, g.dst, scale=10.0, group_size=group)
-        if shape is not None:
-            eng.set_launch_config(*shape)
+        if sh is not None:
+            eng.set_launch_config(*sh)
             c = eng.launch_config()
-            assert (c["walk_block"], c["blocks_per_cu"], c["single_stream"]) == shape
+            assert (c["walk_block"], c["blocks_per_cu"], c["attraction_stream"], c["attraction_blocks_per_cu"]) == sh
         eng.set_positions(pos)
         eng.step(20)
         eng.sync()
@@ -233,30 +234,32 @@ def test_launch_shapes_do_not_change_results(walk_block, blocks_per_cu, single_s
 def test_invalid_launch_config_is_refused():
     g, pos = make_case("tiny", 3)
     eng = Engine(g.n, g.src, g.dst)
-    for bad in ((64, 4, 2), (64, 4, -1), (96, 4, 0), (64, 0, 0), (64, 65, 0)):
+    for bad in ((64, 4, 3), (64, 4, -1), (96, 4, 0), (64, 0, 0), (64, 65, 0), (64, 4, 2, -1), (64, 4, 2, 65)):
         with pytest.raises(GFA2Error):
             eng.set_launch_config(*bad)
     eng.close()
 
 
-def test_streams_in_a_captured_graph_replay_bit_for_bit():
-    """The iteration with the attraction on the second stream, eager and captured as one
-    hipGraph (the second stream forks from and joins the captured stream, replayed 15 times
-    with host synchronisations between the replays), against eager single-stream launches;
-    per-group timing works with two streams."""
-    print("case two streams in a hipGraph", flush=True)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_streams_in_a_captured_graph_replay_bit_for_bit(mode):
+    """The iteration with the attraction on the second stream (beside the walk, or beside keys,
+    sort and tree with a reduced attraction grid), eager and captured as one hipGraph (the
+    second stream forks from and joins the captured stream; replayed 15 times with host
+    synchronisations between the replays), against eager one-stream launches; per-group timing
+    works with two streams."""
+    print(f"case two streams (mode {mode}) in a hipGraph", flush=True)
     g, pos = make_case("rmat", 44)
     out = []
-    for capture, single in ((False, 1), (True, 0), (False, 0)):
+    for capture, m in ((False, 0), (True, mode), (False, mode)):
         eng = Engine(g.n, g.src, g.dst, scale=10.0)
-        eng.set_launch_config(256, 4, single)
+        eng.set_launch_config(256, 4, m, 1 if m == 2 else 0)
         eng.set_graph_capture(capture)
         eng.set_positions(pos)
         for _ in range(3):
             eng.step(5)
             eng.sync()
         out.append(eng.positions())
-        if not capture and not single:
+        if not capture and m:
             eng.set_timing(True)
             eng.step(3)
             eng.sync()
