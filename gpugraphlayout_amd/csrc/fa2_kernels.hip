@@ -276,7 +276,7 @@ __device__ __forceinline__ int xcd_block() {
 // The epilogue adds gravity + attraction, writes the total force, and forms the group's
 // swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
 // ---------------------------------------------------------------------------------------
-constexpr int kWalkStack = 1024;    // stack entries (record indices) per wave
+constexpr int kWalkStack = 512;     // stack entries (record indices) per wave
 constexpr int kWalkSlack = 128;     // the depth-first rounds' headroom: at most 3 per tree level
 constexpr int kWalkList = 256;      // accepted records buffered per wave
 static_assert(kWalkSlack >= 3 * (kMaxLevels + 1) + 4, "a depth-first descent must fit the headroom");
@@ -309,12 +309,12 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     constexpr int kUnroll = 4;                     // drain steps per loop trip
     constexpr int kPad = kStride * kUnroll;
     __shared__ int stk_all[W][kWalkStack];
-    __shared__ float4 lst_all[W][kWalkList + kPad];
+    __shared__ float3 lst_all[W][kWalkList + kPad];
     if (st->err != 0) return;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: one group per wave
     int* stk = stk_all[wv];
-    float4* lst = lst_all[wv];
+    float3* lst = lst_all[wv];
     const int gidx = xcd_block() * W + wv;
     if (gidx >= ngroups) return;                  // uniform per wave; no block barriers below
     const int2 grp = groups[gidx];
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         if (accept) {
             const int ra =
                 __builtin_amdgcn_mbcnt_hi((unsigned)(am >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)am, 0u));
-            lst[nlist + ra] = make_float4(c.x, c.y, k_r * c.z, 0.0f);
+            lst[nlist + ra] = make_float3(c.x, c.y, k_r * c.z);
         }
         top = p3 + __popcll(m3);
         nlist += na;
@@ -434,12 +434,12 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         if (nlist > kWalkList - 64 || top == 0) {
             // pad to whole loop trips with massless entries (their terms are exactly 0)
             const int padded = (nlist + kPad - 1) / kPad * kPad;
-            if (nlist + lane < padded) lst[nlist + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (nlist + lane < padded) lst[nlist + lane] = make_float3(0.0f, 0.0f, 0.0f);
             wave_lds_sync();
             for (int q0 = 0; q0 < padded; q0 += kPad) {
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u) {
-                    const float4 e = lst[q0 + u * kStride + half];
+                    const float3 e = lst[q0 + u * kStride + half];
                     const float dx = px - e.x, dy = py - e.y;
                     const float w = e.z * __builtin_amdgcn_rcpf(fmaf(dx, dx, fmaf(dy, dy, epssq)));
                     ax = fmaf(dx, w, ax);
