@@ -425,12 +425,13 @@ def test_sort_fixup_when_the_tree_deepens(oracle):
     eng.close()
 
 
-@pytest.mark.parametrize("n", [150_000, 600_000, 1_000_000])
-def test_large_sorts_match_numpy_order(n):
-    """Body orders at the sizes of the one-launch large-tile sort passes (up to 128 tiles of
-    8,192 pairs) and beyond, on the first iteration (every key bit sorted) and on later ones
-    (low digit passes skipped by the previous depth), against a numpy sort of the restated keys."""
-    print(f"case large sort n={n}", flush=True)
+@pytest.mark.parametrize("n", [3, 2000, 131_072, 131_073, 150_000, 600_000, 1_000_000])
+def test_sorts_match_numpy_order(n):
+    """Body orders on both sort paths — one launch per pass with look-back (up to 64 tiles of
+    2,048 pairs), three launches per pass — and at their edge, on the first iteration (every key
+    bit sorted) and on later ones (low digit passes skipped by the previous depth), against a
+    numpy sort of the restated keys."""
+    print(f"case sort n={n}", flush=True)
     rng = np.random.default_rng(n)
     pos = rng.uniform(-1000, 1000, size=(n, 2)).astype(np.float32)
     pos[: n // 10] *= np.float32(1e-3)        # a dense core: deeper cells there
