@@ -419,6 +419,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum
         st->level_count[tid] = 0;
         st->level_cursor[tid] = 0;
     }
+    if (tid < 8) st->runmax[tid] = 0;
 }
 
 // S (exclusive prefix of opened cells), body records and brec.
@@ -512,7 +513,9 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
                                                         unsigned char* __restrict__ big_lvl) {
     __shared__ int hist[kMaxLevels + 1];
     __shared__ int lmax;
+    __shared__ int rmax[8];          // largest cell at level 4, 8, ..., 32 seen by the block
     if (threadIdx.x <= kMaxLevels) hist[threadIdx.x] = 0;
+    if (threadIdx.x < 8) rmax[threadIdx.x] = 0;
     if (threadIdx.x == 0) lmax = 0;
     __syncthreads();
     const bool ok = st->err == 0;
@@ -567,6 +570,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
             rec[rc].w = __uint_as_float(info);
         }
         rcount[rc] = b - i + 1;
+        if (L > 0 && (L & 3) == 0 && L <= 32) atomicMax(&rmax[(L >> 2) - 1], b - i + 1);
         if (!pmass) {
             // the per-level cell lists of the summary launches (not needed for integer masses)
             cell_rec[ord] = rc;
@@ -579,6 +583,10 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
     __syncthreads();
     if (!pmass && threadIdx.x <= kMaxLevels && hist[threadIdx.x] != 0)
         atomicAdd(&st->level_count[threadIdx.x], hist[threadIdx.x]);
+    // as for max_level: only blocks above the value seen so far touch the shared words
+    if (threadIdx.x < 8 && rmax[threadIdx.x] >
+                               __hip_atomic_load(&st->runmax[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&st->runmax[threadIdx.x], rmax[threadIdx.x]);
     // every block would hit this one word: only blocks above the value seen so far update it
     // (the maximum only grows, so skipping on a stale value is still correct)
     if (threadIdx.x == 0 && lmax > 0 &&

@@ -50,6 +50,7 @@ struct DevState {
     int nbbox;          // number of valid bounding-box partials
     int nbig;           // heavy cells listed for the in-order summary
     int sort_skip;      // low 8-bit digit passes the next path-key sort skips (0: a full sort)
+    int runmax[8];      // largest body count of a cell at level 4, 8, ..., 32 (this tree)
     int level_count[kMaxLevels + 1];
     int level_offset[kMaxLevels + 2];
     int level_cursor[kMaxLevels + 1];
@@ -211,7 +212,13 @@ constexpr int kTreeTile = kTreeThreads * kTreeItems;
 // whole key (and value), which gives the full sort's order.
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s,
                       const DevState* skip = nullptr);
+// After radix_sort_pairs with a device skip: orders the runs of pairs that agree in every
+// sorted bit by (whole key, value) and leaves the whole result in keys[0]/vals[0] (the passes
+// that ran may have ended in the other buffer).
 void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s);
+// Passes of the path-key sort below which the fix-up finishes the order: the smallest p whose
+// 4p-level cells of the previous tree hold at most this many bodies.
+constexpr int kSortRunMax = 16;
 // Sorts each segment [node_off[g], node_off[g+1]) of buf.keys[0]/vals[0] in place by the full
 // 64-bit key, stable, one block per segment (taken in `order`); for batches whose graphs all
 // have at most kSegSortMax nodes.

@@ -635,10 +635,13 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restr
             st->interactions = ri[0];
             st->global_speed = gs;
             st->speed_efficiency = se;
-            // the next path-key sort: bodies are told apart by the top 2 (max_level + 1) key bits;
-            // two levels of headroom, then whole pairs of 8-bit passes below that are skipped
-            const int need = 2 * (st->max_level + 3);
-            st->sort_skip = need >= 64 ? 0 : 2 * ((64 - need) / 16);
+            // the next path-key sort: p 8-bit passes order the bodies by their top 4p levels,
+            // and the pairs left tied are the bodies of one 4p-level cell, which the fix-up
+            // orders; take the fewest passes whose cells (in this tree) hold at most
+            // kSortRunMax bodies each
+            int p = 1;
+            while (p < 8 && st->runmax[p - 1] > kSortRunMax) ++p;
+            st->sort_skip = 8 - p;
         }
     }
     // per graph (one block each): no word shared by the blocks of a batch

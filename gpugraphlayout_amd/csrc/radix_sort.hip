@@ -27,10 +27,18 @@ __device__ __forceinline__ bool pass_skipped(const DevState* __restrict__ st, in
     return st != nullptr && pass < st->sort_skip;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_upsweep(const unsigned long long* __restrict__ keys,
+// The buffer a pass reads (it writes the other one): the first pass that runs reads buffer 0,
+// and every pass that runs flips it (skipped passes move nothing).
+__device__ __forceinline__ int pass_src(const DevState* __restrict__ st, int pass) {
+    return (pass - (st != nullptr ? st->sort_skip : 0)) & 1;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_upsweep(const unsigned long long* __restrict__ k0,
+                                                          const unsigned long long* __restrict__ k1,
                                                           int n, int shift, unsigned int* __restrict__ hist,
                                                           int tiles, const DevState* __restrict__ st, int pass) {
     if (pass_skipped(st, pass)) return;
+    const unsigned long long* __restrict__ keys = pass_src(st, pass) ? k1 : k0;
     __shared__ unsigned int h[256];
     const int tid = threadIdx.x;
     h[tid] = 0;
@@ -205,14 +213,19 @@ __device__ __forceinline__ unsigned int scatter_tile(const unsigned long long (&
     return tcount;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_scatter(const unsigned long long* __restrict__ kin,
-                                                          const unsigned int* __restrict__ vin,
-                                                          unsigned long long* __restrict__ kout,
-                                                          unsigned int* __restrict__ vout, int n, int shift,
+__global__ __launch_bounds__(kSortThreads) void k_scatter(unsigned long long* __restrict__ k0,
+                                                          unsigned int* __restrict__ v0,
+                                                          unsigned long long* __restrict__ k1,
+                                                          unsigned int* __restrict__ v1, int n, int shift,
                                                           const unsigned int* __restrict__ hist, int tiles,
                                                           const unsigned int* __restrict__ rowtot,
                                                           const DevState* __restrict__ st, int pass) {
     if (pass_skipped(st, pass)) return;
+    const int src = pass_src(st, pass);
+    const unsigned long long* __restrict__ kin = src ? k1 : k0;
+    const unsigned int* __restrict__ vin = src ? v1 : v0;
+    unsigned long long* __restrict__ kout = src ? k0 : k1;
+    unsigned int* __restrict__ vout = src ? v0 : v1;
     __shared__ TileLds L;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -329,16 +342,21 @@ __global__ __launch_bounds__(256) void k_sort_hist(const unsigned long long* __r
     }
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(const unsigned long long* __restrict__ kin,
-                                                           const unsigned int* __restrict__ vin,
-                                                           unsigned long long* __restrict__ kout,
-                                                           unsigned int* __restrict__ vout, int n, int shift,
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(unsigned long long* __restrict__ k0,
+                                                           unsigned int* __restrict__ v0,
+                                                           unsigned long long* __restrict__ k1,
+                                                           unsigned int* __restrict__ v1, int n, int shift,
                                                            const unsigned int* __restrict__ digit_hist,
                                                            unsigned int* __restrict__ status,
                                                            unsigned long long* __restrict__ ticket, int tiles,
                                                            int* __restrict__ err, const DevState* __restrict__ st,
                                                            int pass) {
     if (pass_skipped(st, pass)) return;   // takes no ticket: the epochs of later passes stay distinct
+    const int src = pass_src(st, pass);
+    const unsigned long long* __restrict__ kin = src ? k1 : k0;
+    const unsigned int* __restrict__ vin = src ? v1 : v0;
+    unsigned long long* __restrict__ kout = src ? k0 : k1;
+    unsigned int* __restrict__ vout = src ? v0 : v1;
     __shared__ TileLds L;
     __shared__ unsigned int tcount[256];
     __shared__ int s_tile;
@@ -435,38 +453,50 @@ __device__ void heapsort_pairs(unsigned long long* __restrict__ k, unsigned int*
     }
 }
 
-// After a sort that skipped its low passes: every run of pairs whose keys agree in the sorted
-// top bits is ordered by (whole key, value) in place by the thread that owns the run's first
-// position (insertion sort up to 32 pairs, heapsort beyond). Such runs exist only where bodies
-// share more quadrant levels than the skip allowed for (the deepest cell grew since the previous
-// iteration, or bodies coincide), so they are few and short.
-__global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restrict__ keys,
-                                                    unsigned int* __restrict__ vals, int n,
+// After a sort that skipped its low passes: the passes that ran left the pairs ordered by
+// their top 8 (8 - skip) key bits, in buffer (8 - skip) & 1. Every run of pairs whose keys agree
+// in those bits is ordered by (whole key, value) by the thread that owns the run's first
+// position (insertion sort up to 32 pairs, heapsort beyond), and the result goes to buffer 0
+// (pairs outside runs are copied there when the passes ended in buffer 1). Such runs are the
+// bodies of one cell at the sorted depth, few and short when the depth was chosen from the
+// previous tree.
+__global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restrict__ k0,
+                                                    unsigned int* __restrict__ v0,
+                                                    const unsigned long long* __restrict__ k1,
+                                                    const unsigned int* __restrict__ v1, int n,
                                                     const DevState* __restrict__ st) {
     const int skip = st->sort_skip;
-    if (skip == 0) return;
+    if (skip == 0) return;                       // all 8 passes ran: sorted, ties by value, in buffer 0
+    const bool moved = ((8 - skip) & 1) != 0;    // the passes that ran ended in buffer 1
+    const unsigned long long* __restrict__ ks = moved ? k1 : k0;
+    const unsigned int* __restrict__ vs = moved ? v1 : v0;
     const unsigned long long top = ~0ull << (8 * skip);
-    for (int i = blockIdx.x * 256 + threadIdx.x; i + 1 < n; i += gridDim.x * 256) {
-        const unsigned long long ki = keys[i] & top;
-        if ((keys[i + 1] & top) != ki) continue;
-        if (i > 0 && (keys[i - 1] & top) == ki) continue;     // not the first of its run
-        int j = i + 2;
-        while (j < n && (keys[j] & top) == ki) ++j;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const unsigned long long ki = ks[i] & top;
+        if (i > 0 && (ks[i - 1] & top) == ki) continue;     // not the first of its run
+        int j = i + 1;
+        while (j < n && (ks[j] & top) == ki) ++j;
+        if (moved)
+            for (int a = i; a < j; ++a) {
+                k0[a] = ks[a];
+                v0[a] = vs[a];
+            }
+        if (j - i == 1) continue;
         if (j - i > 32) {
-            heapsort_pairs(keys + i, vals + i, j - i);
+            heapsort_pairs(k0 + i, v0 + i, j - i);
             continue;
         }
         for (int a = i + 1; a < j; ++a) {
-            const unsigned long long k = keys[a];
-            const unsigned int v = vals[a];
+            const unsigned long long k = k0[a];
+            const unsigned int v = v0[a];
             int b = a - 1;
-            while (b >= i && pair_less(k, v, keys[b], vals[b])) {
-                keys[b + 1] = keys[b];
-                vals[b + 1] = vals[b];
+            while (b >= i && pair_less(k, v, k0[b], v0[b])) {
+                k0[b + 1] = k0[b];
+                v0[b + 1] = v0[b];
                 --b;
             }
-            keys[b + 1] = k;
-            vals[b + 1] = v;
+            k0[b + 1] = k;
+            v0[b + 1] = v;
         }
     }
 }
@@ -474,10 +504,10 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
 }  // namespace
 
 void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s) {
-    if (n <= 1) return;
+    if (n < 1) return;
     int blocks = (n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
-    k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], n, st);
+    k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], n, st);
 }
 
 void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s) {
@@ -488,35 +518,32 @@ void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* orde
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s, const DevState* skip) {
     if (n <= 1) return;
     const int tiles = (n + kSortTile - 1) / kSortTile;
-    int cur = 0;
+    const int npasses = (end_bit - begin_bit + 7) / 8;
+    // Every pass picks its buffers on the device (pass_src): with a device skip the passes that
+    // run may end in buffer 1, and launch_sort_fixup moves the result to buffer 0.
+    unsigned long long *k0 = buf.keys[0], *k1 = buf.keys[1];
+    unsigned int *v0 = buf.vals[0], *v1 = buf.vals[1];
     // One launch per pass pays off while the look-back chains stay short; with hundreds of
     // tiles resident at once the chains grow with the tile count and three launches per pass win.
     if (buf.onesweep && tiles <= kOnesweepMaxTiles) {
-        const int npasses = (end_bit - begin_bit + 7) / 8;
         k_sort_zero<<<1, 256, 0, s>>>(buf.digit_hist, npasses);
         int hb = (n + 255) / 256;
         if (hb > 256) hb = 256;
-        k_sort_hist<<<hb, 256, 0, s>>>(buf.keys[0], n, begin_bit, npasses, buf.digit_hist);
-        for (int p = 0; p < npasses; ++p) {
-            k_onesweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], buf.vals[cur], buf.keys[cur ^ 1],
-                                                      buf.vals[cur ^ 1], n, begin_bit + 8 * p,
-                                                      buf.digit_hist + 256 * p, buf.status, buf.ticket, tiles,
-                                                      buf.err, skip, p);
-            cur ^= 1;
-        }
+        k_sort_hist<<<hb, 256, 0, s>>>(k0, n, begin_bit, npasses, buf.digit_hist);
+        for (int p = 0; p < npasses; ++p)
+            k_onesweep<<<tiles, kSortThreads, 0, s>>>(k0, v0, k1, v1, n, begin_bit + 8 * p, buf.digit_hist + 256 * p,
+                                                      buf.status, buf.ticket, tiles, buf.err, skip, p);
     } else {
-        for (int shift = begin_bit, p = 0; shift < end_bit; shift += 8, ++p) {
-            k_upsweep<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], n, shift, buf.hist, tiles, skip, p);
+        for (int p = 0; p < npasses; ++p) {
+            const int shift = begin_bit + 8 * p;
+            k_upsweep<<<tiles, kSortThreads, 0, s>>>(k0, k1, n, shift, buf.hist, tiles, skip, p);
             k_rowscan<<<256, 256, 0, s>>>(buf.hist, tiles, buf.rowtot, skip, p);
-            k_scatter<<<tiles, kSortThreads, 0, s>>>(buf.keys[cur], buf.vals[cur], buf.keys[cur ^ 1],
-                                                     buf.vals[cur ^ 1], n, shift, buf.hist, tiles, buf.rowtot, skip,
-                                                     p);
-            cur ^= 1;
+            k_scatter<<<tiles, kSortThreads, 0, s>>>(k0, v0, k1, v1, n, shift, buf.hist, tiles, buf.rowtot, skip, p);
         }
     }
-    if (cur != 0) {
-        hipMemcpyAsync(buf.keys[0], buf.keys[1], sizeof(unsigned long long) * n, hipMemcpyDeviceToDevice, s);
-        hipMemcpyAsync(buf.vals[0], buf.vals[1], sizeof(unsigned int) * n, hipMemcpyDeviceToDevice, s);
+    if (!skip && (npasses & 1)) {
+        hipMemcpyAsync(k0, k1, sizeof(unsigned long long) * n, hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(v0, v1, sizeof(unsigned int) * n, hipMemcpyDeviceToDevice, s);
     }
 }
 

@@ -378,12 +378,23 @@ def expected_order(p):
     return np.lexsort((np.arange(len(p)), key)), key
 
 
+def sorted_passes(T):
+    """The digit passes the next path-key sort runs, restated from a tree: the fewest p whose
+    level-4p cells hold at most 16 bodies each (the pairs still tied after p passes are the
+    bodies of one level-4p cell, which the fix-up orders)."""
+    for p in range(1, 8):
+        cnt = T["count"][T["level"] == 4 * p]
+        if len(cnt) == 0 or cnt.max() <= 16:
+            return p
+    return 8
+
+
 def test_sort_fixup_when_the_tree_deepens(oracle):
-    """The path-key sort skips low digit passes by the previous tree's depth; when the tree
-    deepens past that headroom in one iteration (pairs of bodies joined by an edge pull
-    together), the fix-up must still give the full sort's order. Every iteration's body order
-    equals a numpy sort of the restated keys, its tree is the oracle's where no bodies
-    coincide, and at least one iteration needed the fix-up."""
+    """The path-key sort runs only the top digit passes the previous tree's cells call for; when
+    the tree deepens in one iteration (pairs of bodies joined by an edge pull together), the
+    fix-up must still give the full sort's order. Every iteration's body order equals a numpy
+    sort of the restated keys, its tree is the oracle's where no bodies coincide, and at least
+    one iteration needed the fix-up."""
     print("case sort fix-up on a deepening tree", flush=True)
     rng = np.random.default_rng(12)
     npairs = 600
@@ -396,7 +407,7 @@ def test_sort_fixup_when_the_tree_deepens(oracle):
     # every collapsing pair hangs a chain of cells down to the level where it parts: room for them
     eng = Engine(2 * npairs, src, dst, scale=1e-6, gravity=0.0, strong_gravity=1, cell_factor=40)
     eng.set_positions(pos)
-    prev_level, exercised, checked = None, 0, 0
+    prev_tree, exercised, checked = None, 0, 0
     for it in range(6):
         p = eng.positions()
         eng.step(1)
@@ -405,14 +416,13 @@ def test_sort_fixup_when_the_tree_deepens(oracle):
         st = eng.stats()
         order, key = expected_order(p)
         np.testing.assert_array_equal(T["perm"], order, err_msg=f"iteration {it}")
-        if prev_level is not None:
-            # the bits this iteration's sort ordered by: see the sort skip in the speed step
-            need = 2 * (prev_level + 3)
-            bits = 64 - 8 * (0 if need >= 64 else 2 * ((64 - need) // 16))
+        if prev_tree is not None:
+            # the bits this iteration's sort ordered by: chosen from the previous tree
+            bits = 8 * sorted_passes(prev_tree)
             k = key[order]
             shift = np.uint64(64 - bits)
             exercised += int(bits < 64 and np.any((k[1:] >> shift) == (k[:-1] >> shift)))
-        prev_level = st["max_level"]
+        prev_tree = T
         print(f"iteration {it}: max level {st['max_level']}, cells {st['cells']}", flush=True)
         if len(np.unique(p, axis=0)) < len(p):
             continue        # coincident bodies: the reference nudges them, this engine does not
