@@ -66,6 +66,9 @@ struct gfa2_engine {
     // graphs of a batch (one for a single graph)
     std::vector<int> node_off_host;
     int groups_cap = 0;
+    // the lowest path-key sort pass launched: one below the lowest the device ran when the host
+    // last read its state (a full sort after new positions or parameters)
+    int sort_first = 0;
     // launch shapes (autotuned per graph-size bucket)
     int cus = 256;
     LaunchConfig tune{256, 8, 0, 0};
@@ -201,8 +204,8 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
         } else {
             // the previous tree's depth bounds the key bits the order needs (skipped low passes
             // are made up for by the fix-up, so the order is the full sort's whatever the depth)
-            radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st);
-            launch_sort_fixup(b.sort, b.n, b.st, s);
+            radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st, h->sort_first);
+            launch_sort_fixup(b.sort, b.n, b.st, s, h->sort_first);
             if (b.nseg > 1) launch_seg_graph_sort(b, s);
         }
     }
@@ -271,7 +274,23 @@ int build_groups(gfa2_engine* h);
 
 // New positions or parameters: the previous tree's depth says nothing about the next one, so
 // the next path-key sort sorts every key bit.
+void invalidate_graph(gfa2_engine* h) {
+    h->graph_valid = false;
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+}
+
+// The passes the host launches follow the device's choice, as last read, with one pass to
+// spare (fewer passes than the device asks for only lengthen the fix-up's runs; the order is
+// the same). A change of launches needs a new captured graph.
+void set_sort_first(gfa2_engine* h, int first) {
+    if (first == h->sort_first) return;
+    h->sort_first = first;
+    invalidate_graph(h);
+}
+
 hipError_t full_sort_next(gfa2_engine* h) {
+    set_sort_first(h, 0);
     return hipMemsetAsync(&h->b.st->sort_skip, 0, sizeof(int), h->stream);
 }
 
@@ -287,6 +306,7 @@ int device_error(int err) {
 int read_state(gfa2_engine* h, DevState* st) {
     GFA2_TRY(hipMemcpyAsync(st, h->b.st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
     GFA2_TRY(hipStreamSynchronize(h->stream));
+    set_sort_first(h, std::max(0, std::min(st->sort_skip, 7) - 1));
     return GFA2_OK;
 }
 

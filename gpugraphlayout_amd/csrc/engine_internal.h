@@ -207,15 +207,16 @@ constexpr int kTreeTile = kTreeThreads * kTreeItems;
 
 // Radix sort of (key, value) pairs on 64-bit keys, ascending and stable. Sorts
 // buf.keys[0]/vals[0] in place (via buf.keys[1]/vals[1]). With skip (path keys only, begin_bit
-// 0), the first skip->sort_skip passes (an even number) do nothing: the pairs end up sorted by
-// the key's top bits only, and launch_sort_fixup then orders each run of equal top bits by the
-// whole key (and value), which gives the full sort's order.
+// 0), the passes below skip->sort_skip do nothing and the passes below `first` are not
+// launched: the pairs end up sorted by the key's top bits only, possibly in buffer 1, and
+// launch_sort_fixup (given the same `first`) then orders each run of equal top bits by the
+// whole key (and value), which gives the full sort's order, in buffer 0.
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s,
-                      const DevState* skip = nullptr);
+                      const DevState* skip = nullptr, int first = 0);
 // After radix_sort_pairs with a device skip: orders the runs of pairs that agree in every
 // sorted bit by (whole key, value) and leaves the whole result in keys[0]/vals[0] (the passes
 // that ran may have ended in the other buffer).
-void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s);
+void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first = 0);
 // Passes of the path-key sort below which the fix-up finishes the order: the smallest p whose
 // 4p-level cells of the previous tree hold at most this many bodies.
 constexpr int kSortRunMax = 16;

@@ -27,18 +27,24 @@ __device__ __forceinline__ bool pass_skipped(const DevState* __restrict__ st, in
     return st != nullptr && pass < st->sort_skip;
 }
 
+// The low passes that do not run: those the device skips and those the host did not launch.
+__device__ __forceinline__ int passes_below(const DevState* __restrict__ st, int first) {
+    return max(st != nullptr ? st->sort_skip : 0, first);
+}
+
 // The buffer a pass reads (it writes the other one): the first pass that runs reads buffer 0,
 // and every pass that runs flips it (skipped passes move nothing).
-__device__ __forceinline__ int pass_src(const DevState* __restrict__ st, int pass) {
-    return (pass - (st != nullptr ? st->sort_skip : 0)) & 1;
+__device__ __forceinline__ int pass_src(const DevState* __restrict__ st, int first, int pass) {
+    return (pass - passes_below(st, first)) & 1;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_upsweep(const unsigned long long* __restrict__ k0,
                                                           const unsigned long long* __restrict__ k1,
                                                           int n, int shift, unsigned int* __restrict__ hist,
-                                                          int tiles, const DevState* __restrict__ st, int pass) {
+                                                          int tiles, const DevState* __restrict__ st, int first,
+                                                          int pass) {
     if (pass_skipped(st, pass)) return;
-    const unsigned long long* __restrict__ keys = pass_src(st, pass) ? k1 : k0;
+    const unsigned long long* __restrict__ keys = pass_src(st, first, pass) ? k1 : k0;
     __shared__ unsigned int h[256];
     const int tid = threadIdx.x;
     h[tid] = 0;
@@ -219,9 +225,9 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(unsigned long long* __
                                                           unsigned int* __restrict__ v1, int n, int shift,
                                                           const unsigned int* __restrict__ hist, int tiles,
                                                           const unsigned int* __restrict__ rowtot,
-                                                          const DevState* __restrict__ st, int pass) {
+                                                          const DevState* __restrict__ st, int first, int pass) {
     if (pass_skipped(st, pass)) return;
-    const int src = pass_src(st, pass);
+    const int src = pass_src(st, first, pass);
     const unsigned long long* __restrict__ kin = src ? k1 : k0;
     const unsigned int* __restrict__ vin = src ? v1 : v0;
     unsigned long long* __restrict__ kout = src ? k0 : k1;
@@ -350,9 +356,9 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(unsigned long long* _
                                                            unsigned int* __restrict__ status,
                                                            unsigned long long* __restrict__ ticket, int tiles,
                                                            int* __restrict__ err, const DevState* __restrict__ st,
-                                                           int pass) {
+                                                           int first, int pass) {
     if (pass_skipped(st, pass)) return;   // takes no ticket: the epochs of later passes stay distinct
-    const int src = pass_src(st, pass);
+    const int src = pass_src(st, first, pass);
     const unsigned long long* __restrict__ kin = src ? k1 : k0;
     const unsigned int* __restrict__ vin = src ? v1 : v0;
     unsigned long long* __restrict__ kout = src ? k0 : k1;
@@ -464,8 +470,8 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
                                                     unsigned int* __restrict__ v0,
                                                     const unsigned long long* __restrict__ k1,
                                                     const unsigned int* __restrict__ v1, int n,
-                                                    const DevState* __restrict__ st) {
-    const int skip = st->sort_skip;
+                                                    const DevState* __restrict__ st, int first) {
+    const int skip = passes_below(st, first);
     if (skip == 0) return;                       // all 8 passes ran: sorted, ties by value, in buffer 0
     const bool moved = ((8 - skip) & 1) != 0;    // the passes that ran ended in buffer 1
     const unsigned long long* __restrict__ ks = moved ? k1 : k0;
@@ -503,11 +509,11 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
 
 }  // namespace
 
-void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s) {
+void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first) {
     if (n < 1) return;
     int blocks = (n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
-    k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], n, st);
+    k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], n, st, first);
 }
 
 void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s) {
@@ -515,7 +521,8 @@ void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* orde
         k_seg_sort<<<nseg, kSortThreads, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], node_off, order);
 }
 
-void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s, const DevState* skip) {
+void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s, const DevState* skip,
+                      int first) {
     if (n <= 1) return;
     const int tiles = (n + kSortTile - 1) / kSortTile;
     const int npasses = (end_bit - begin_bit + 7) / 8;
@@ -530,18 +537,19 @@ void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipSt
         int hb = (n + 255) / 256;
         if (hb > 256) hb = 256;
         k_sort_hist<<<hb, 256, 0, s>>>(k0, n, begin_bit, npasses, buf.digit_hist);
-        for (int p = 0; p < npasses; ++p)
+        for (int p = first; p < npasses; ++p)
             k_onesweep<<<tiles, kSortThreads, 0, s>>>(k0, v0, k1, v1, n, begin_bit + 8 * p, buf.digit_hist + 256 * p,
-                                                      buf.status, buf.ticket, tiles, buf.err, skip, p);
+                                                      buf.status, buf.ticket, tiles, buf.err, skip, first, p);
     } else {
-        for (int p = 0; p < npasses; ++p) {
+        for (int p = first; p < npasses; ++p) {
             const int shift = begin_bit + 8 * p;
-            k_upsweep<<<tiles, kSortThreads, 0, s>>>(k0, k1, n, shift, buf.hist, tiles, skip, p);
+            k_upsweep<<<tiles, kSortThreads, 0, s>>>(k0, k1, n, shift, buf.hist, tiles, skip, first, p);
             k_rowscan<<<256, 256, 0, s>>>(buf.hist, tiles, buf.rowtot, skip, p);
-            k_scatter<<<tiles, kSortThreads, 0, s>>>(k0, v0, k1, v1, n, shift, buf.hist, tiles, buf.rowtot, skip, p);
+            k_scatter<<<tiles, kSortThreads, 0, s>>>(k0, v0, k1, v1, n, shift, buf.hist, tiles, buf.rowtot, skip, first,
+                                                     p);
         }
     }
-    if (!skip && (npasses & 1)) {
+    if (!skip && ((npasses - first) & 1)) {
         hipMemcpyAsync(k0, k1, sizeof(unsigned long long) * n, hipMemcpyDeviceToDevice, s);
         hipMemcpyAsync(v0, v1, sizeof(unsigned int) * n, hipMemcpyDeviceToDevice, s);
     }
