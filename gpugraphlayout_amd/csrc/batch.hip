@@ -50,11 +50,12 @@ __global__ __launch_bounds__(256) void k_seg_bbox(const float2* __restrict__ pos
     }
 }
 
-__global__ __launch_bounds__(256) void k_seg_keys(const float2* __restrict__ pos, int n, const int* __restrict__ seg_of,
+__global__ __launch_bounds__(256) void k_seg_keys(const float2* __restrict__ pos, const float* __restrict__ mass, int n,
+                                                  const int* __restrict__ seg_of,
                                                   const float4* __restrict__ seg_root,
                                                   unsigned long long* __restrict__ keys,
                                                   unsigned int* __restrict__ vals,
-                                                  unsigned long long* __restrict__ pkey) {
+                                                  unsigned long long* __restrict__ pkey, float4* __restrict__ pm) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float4 root = seg_root[seg_of[i]];
         const float2 p = pos[i];
@@ -75,6 +76,7 @@ __global__ __launch_bounds__(256) void k_seg_keys(const float2* __restrict__ pos
         keys[i] = key;
         if (pkey) pkey[i] = key;      // kept for the regather after a global graph sort
         vals[i] = (unsigned)i;
+        pm[i] = make_float4(p.x, p.y, mass[i], 0.0f);
     }
 }
 
@@ -136,8 +138,8 @@ void launch_seg_bbox(const Bufs& b, hipStream_t s) {
 }
 
 void launch_seg_keys(const Bufs& b, hipStream_t s) {
-    k_seg_keys<<<grid_for(b.n), 256, 0, s>>>(b.pos, b.n, b.seg_of, b.seg_root, b.sort.keys[0], b.sort.vals[0],
-                                             b.max_seg <= kSegSortMax ? nullptr : b.pkey);
+    k_seg_keys<<<grid_for(b.n), 256, 0, s>>>(b.pos, b.mass, b.n, b.seg_of, b.seg_root, b.sort.keys[0], b.sort.vals[0],
+                                             b.max_seg <= kSegSortMax ? nullptr : b.pkey, b.pm);
 }
 
 void launch_seg_graph_sort(Bufs& b, hipStream_t s) {

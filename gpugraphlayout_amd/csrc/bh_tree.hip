@@ -99,12 +99,13 @@ __global__ __launch_bounds__(256) void k_bbox_partials(const float2* __restrict_
     }
 }
 
-// Root box + path keys. vals = identity.
-__global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, int n,
+// Root box + path keys. vals = identity; pm = {x, y, mass} per node, so that k_lcp gathers one
+// 16-byte record per sorted body instead of a position and a mass.
+__global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, const float* __restrict__ mass, int n,
                                               const float4* __restrict__ part, DevState* st,
                                               float4* __restrict__ seg_root,
                                               unsigned long long* __restrict__ keys,
-                                              unsigned int* __restrict__ vals) {
+                                              unsigned int* __restrict__ vals, float4* __restrict__ pm) {
     __shared__ float root[3];
     reduce_bbox(part, st->nbbox, root);
     const float rootx = root[0], rooty = root[1], radius = root[2];
@@ -132,6 +133,7 @@ __global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, in
         }
         keys[i] = key;
         vals[i] = (unsigned)i;
+        pm[i] = make_float4(p.x, p.y, mass[i], 0.0f);   // one gather per body for k_lcp
     }
 }
 
@@ -274,8 +276,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
                                                       signed char* __restrict__ lcp,
                                                       int* __restrict__ blocksum,
                                                       const unsigned int* __restrict__ vals,
-                                                      const float2* __restrict__ pos,
-                                                      const float* __restrict__ mass,
+                                                      const float4* __restrict__ pm,
                                                       const float4* __restrict__ seg_root,
                                                       Moment* __restrict__ bmass, float4* __restrict__ sbody) {
     // the tile's keys (with one on each side), graph ids and body ids, read coalesced into
@@ -304,14 +305,9 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
     const int i0 = base + threadIdx.x * kTreeItems;
     // all gathers of the thread's items first, outside the per-item branch, so they are in
     // flight together (ids past n are 0, a valid node)
-    float2 pp[kTreeItems];
-    float mm[kTreeItems];
+    float4 pp[kTreeItems];
 #pragma unroll
-    for (int k = 0; k < kTreeItems; ++k) {
-        const unsigned int id = sval[i0 + k - base];
-        pp[k] = pos[id];
-        mm[k] = mass[id];
-    }
+    for (int k = 0; k < kTreeItems; ++k) pp[k] = pm[sval[i0 + k - base]];
     int sum = 0;
     Moment ms{0ull, 0ull, 0ull, 0ull, 0ull};
 #pragma unroll
@@ -322,7 +318,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
             int l = lcp_at_l(i);
             lcp[i] = (signed char)l;
             sum += max(0, l - prev);
-            const float4 bd = make_float4(pp[k].x, pp[k].y, mm[k], 0.0f);
+            const float4 bd = make_float4(pp[k].x, pp[k].y, pp[k].z, 0.0f);
             sbody[i] = bd;
             if (bmass) ms = mo_add(ms, body_moment(bd, sg ? ssg[i - base + 1] : 0, seg_root));
         }
@@ -570,10 +566,10 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
             rec[rc].w = __uint_as_float(info);
         }
         rcount[rc] = b - i + 1;
+        cell_rec[ord] = rc;       // the cell list of k_kids (and of the summary launches)
         if (L > 0 && (L & 3) == 0 && L <= 32) atomicMax(&rmax[(L >> 2) - 1], b - i + 1);
         if (!pmass) {
             // the per-level cell lists of the summary launches (not needed for integer masses)
-            cell_rec[ord] = rc;
             cell_lvl[ord] = (unsigned char)L;
             atomicAdd(&hist[L], 1);
         }
@@ -598,12 +594,13 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
 // [r, r + skip); its children are r + 1 and then each next record past the previous child's
 // subtree (a body spans one record), at most four. Unused slots hold -1.
 __global__ __launch_bounds__(256) void k_kids(const float4* __restrict__ rec, const int* __restrict__ S, int n,
-                                              int4* __restrict__ kids, const DevState* __restrict__ st) {
+                                              const int* __restrict__ cell_rec, int4* __restrict__ kids,
+                                              const DevState* __restrict__ st) {
     if (st->err != 0) return;
-    const int nrec = n + S[n];
-    for (int r = blockIdx.x * 256 + threadIdx.x; r < nrec; r += gridDim.x * 256) {
+    const int ncells = S[n];
+    for (int ord = blockIdx.x * 256 + threadIdx.x; ord < ncells; ord += gridDim.x * 256) {
+        const int r = cell_rec[ord];
         const unsigned int info = __float_as_uint(rec[r].w);
-        if (info == 0xffffffffu) continue;
         const int end = r + (int)(info >> kInfoLevelBits);
         int k[4] = {-1, -1, -1, -1};
         int c = r + 1;
@@ -777,7 +774,8 @@ void launch_bbox_partials(const Bufs& b, hipStream_t s) {
 void launch_keys(const Bufs& b, hipStream_t s) {
     int blocks = (b.n + 255) / 256;
     if (blocks > b.grid_cap) blocks = b.grid_cap;
-    k_keys<<<blocks, 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st, b.seg_root, b.sort.keys[0], b.sort.vals[0]);
+    k_keys<<<blocks, 256, 0, s>>>(b.pos, b.mass, b.n, b.bbox_part, b.st, b.seg_root, b.sort.keys[0], b.sort.vals[0],
+                                  b.pm);
 }
 
 void launch_tree(Bufs& b, hipStream_t s) {
@@ -786,7 +784,7 @@ void launch_tree(Bufs& b, hipStream_t s) {
     const int* sg = b.nseg > 1 ? b.sg : nullptr;
     Moment* bm = b.int_mass ? b.bmass : nullptr;
     Moment* pm = b.int_mass ? b.pmass : nullptr;
-    k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum, b.sort.vals[0], b.pos, b.mass,
+    k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum, b.sort.vals[0], b.pm,
                                          b.seg_root, bm, b.sbody);
     k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st, bm);
     const int max_cells = b.cap_records - n;
@@ -802,7 +800,7 @@ void launch_tree(Bufs& b, hipStream_t s) {
         // every record's info word is in place (cells: k_nodes, bodies: k_prefix)
         int kb = (2 * n + 255) / 256;
         if (kb > 4096) kb = 4096;
-        k_kids<<<kb, 256, 0, s>>>(b.rec, b.S, n, b.kids, b.st);
+        k_kids<<<kb, 256, 0, s>>>(b.rec, b.S, n, b.cell_rec, b.kids, b.st);
     }
     if (b.int_mass) {
         k_big<<<1, 256, 0, s>>>(b.rec, b.big_rc, b.big_lvl, b.st);

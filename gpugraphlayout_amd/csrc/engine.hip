@@ -551,6 +551,7 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.kids, (size_t)cap);
     GFA2_ALLOC(b.brec, n);
     GFA2_ALLOC(b.sbody, n);
+    GFA2_ALLOC(b.pm, n);
     GFA2_ALLOC(b.rcount, (size_t)cap);
     GFA2_ALLOC(b.cell_rec, maxcells);
     GFA2_ALLOC(b.cell_lvl, maxcells);

@@ -146,6 +146,7 @@ struct Bufs {
     int4* kids;             // per cell record: the record indices of its 1-4 children (-1: none)
     int* brec;
     float4* sbody;          // {x, y, mass, 0} of the body at sorted position k (node k in exact mode)
+    float4* pm;             // {x, y, mass, 0} of node v, written with the keys (k_lcp gathers it)
     float* ew_raw = nullptr;    // edge weights per CSR entry (null: unweighted)
     float* ew = nullptr;        // w^delta per CSR entry
     int* rcount;
