@@ -7,7 +7,9 @@ Workload (default): R-MAT over 2^20 ids, 10,000,000 distinct undirected non-loop
 of side 2*sqrt(V); weak gravity k_g = 1, k_r = 10, theta = 1. One step = one FA2
 iteration (tree + walk + attraction + gravity + speed + displacement), all inputs
 resident in HBM. W untimed warm-up iterations, then K timed ones, bracketed by a device
-synchronisation on both sides.
+synchronisation on both sides: eager launches with HIP events around every kernel group (the
+kernel times of the roofline), or for ba10k, whose iteration is bound by launch latency, K
+replays of the iteration captured as one hipGraph, after an equal eager pass with events.
 
 Also reported:
   roofline      the Barnes-Hut walk's (FP32 VALU: 12 flops per body-node interaction,
@@ -309,6 +311,8 @@ def run_single(args):
     log(f"warm-up done ({args.warmup} it), timing {steps} it")
     dt, groups, s0, s1, launch, tune_s = timed_run(eng, args, steps)
     value = steps / dt
+    events_value = value
+    timed = args.timed or ("replay" if args.workload == "ba10k" else "events")
 
     V, E = g.n, g.m
     sb = streaming_bytes(V, E)
@@ -316,15 +320,23 @@ def run_single(args):
     roof = walk_roofline(groups, s0, s1, steps, V, launch["attraction_stream"] == 1)
 
     replay = None
-    if args.graph_replay > 0:
+    replay_iters = steps if timed == "replay" else args.graph_replay
+    if replay_iters > 0:
+        # the iteration captured once as a hipGraph (one launch per iteration), then replayed
         eng.set_graph_capture(True)
         eng.step(1)
         eng.sync()
         t1 = time.perf_counter()
-        eng.step(args.graph_replay)
+        eng.step(replay_iters)
         eng.sync()
-        replay = args.graph_replay / (time.perf_counter() - t1)
+        replay_dt = time.perf_counter() - t1
+        replay = replay_iters / replay_dt
         eng.set_graph_capture(False)
+        if timed == "replay":
+            # the timed region: K replayed iterations; the kernel groups and the roofline come
+            # from the K eager iterations with HIP events just before
+            dt, value = replay_dt, replay
+            log(f"hipGraph replay (timed region): {value:.1f} it/s")
     final = eng.stats()
 
     # layout quality: a fresh run of the workload's own iteration count from the same start, so
@@ -377,8 +389,12 @@ def run_single(args):
                       dict(cfg, group_size=args.group, parallelism="single-gpu",
                            walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"],
                            attraction_stream=launch["attraction_stream"],
-                           attraction_blocks_per_cu=launch["attraction_blocks_per_cu"]))
+                           attraction_blocks_per_cu=launch["attraction_blocks_per_cu"],
+                           timed=("hipGraph replay, one launch per iteration; kernel groups and roofline from "
+                                  "the same number of eager iterations with HIP events just before"
+                                  if timed == "replay" else "eager launches with HIP events around every group")))
     out.update({
+        "eager_events_iterations_per_s": events_value,
         "roofline": roof,
         "cpu_baseline": cpu,
         "time_to_equal_stress": tte,
@@ -517,6 +533,10 @@ def main():
                     help="with --walk-block: attraction grid per compute unit (0: one block per work item)")
     ap.add_argument("--graph-replay", type=int, default=100,
                     help="extra iterations replayed as one captured hipGraph per iteration (reported apart)")
+    ap.add_argument("--timed", choices=("events", "replay"), default=None,
+                    help="single graphs: the timed region as eager launches with HIP events (default) or as "
+                         "hipGraph replays (default for ba10k, whose iteration is launch-bound), kernel groups "
+                         "then from an equal eager pass")
     ap.add_argument("--traffic", type=int, default=None,
                     help="1: measure the walk's HBM bytes with rocprofv3 counters in child runs (default: on for "
                          "rmat20)")
