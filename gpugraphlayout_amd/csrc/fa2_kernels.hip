@@ -343,7 +343,9 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     const float bmy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), jm));
     const float blx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px), jl));
     const float bly = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), jl));
-    const unsigned long long voters = __builtin_amdgcn_ballot_w64(bact && lane < G);
+    // the lanes whose bodies vote: the group's active bodies (for G = 32 in both half-waves,
+    // which hold the same bodies; each half is read as its own 32-bit vote)
+    const unsigned long long voters = __builtin_amdgcn_ballot_w64(bact);
     const float two_r = seg_root[g].z * 2;
     const float base = two_r * two_r * itolsq;
     // the root's children start the walk (a graph of one body has no cell records)
@@ -386,18 +388,40 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         // the first, middle and last bodies' own d2: any below thr rejects
         const bool pass3 = vote_d2(b0x, b0y, c.x, c.y, epssq) >= thr && vote_d2(bmx, bmy, c.x, c.y, epssq) >= thr &&
                            vote_d2(blx, bly, c.x, c.y, epssq) >= thr;
-        // the remaining cells, voted on by the group's bodies one cell at a time
+        // the remaining cells, voted on by the group's bodies (lanes = bodies); with G = 32 the
+        // two half-waves hold the same bodies and vote on two cells at once
         unsigned long long okm = __builtin_amdgcn_ballot_w64(live && (body || clear));
         unsigned long long um = __builtin_amdgcn_ballot_w64(live && !body && !clear && pass3);
         while (um) {
             const int j = __builtin_ctzll(um);
             um &= um - 1;
-            const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.x), j));
-            const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.y), j));
-            const float tj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr), j));
+            int j2 = j;
+            if constexpr (G == 32) {
+                if (um) {
+                    j2 = __builtin_ctzll(um);
+                    um &= um - 1;
+                }
+            }
+            float cx, cy, tj;
+            if constexpr (G == 32) {
+                // each half-wave fetches its own cell's values (the upper half: the second cell)
+                const int src = (lane >= 32 ? j2 : j) << 2;
+                cx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(c.x)));
+                cy = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(c.y)));
+                tj = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(thr)));
+            } else {
+                cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.x), j));
+                cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.y), j));
+                tj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr), j));
+            }
             const unsigned long long fails =
                 __builtin_amdgcn_ballot_w64(!(vote_d2(px, py, cx, cy, epssq) >= tj)) & voters;
-            if (fails == 0ull) okm |= 1ull << j;
+            if constexpr (G == 32) {
+                if ((unsigned int)fails == 0u) okm |= 1ull << j;
+                if ((unsigned int)(fails >> 32) == 0u) okm |= 1ull << j2;
+            } else {
+                if (fails == 0ull) okm |= 1ull << j;
+            }
         }
         const bool accept = live && ((okm >> lane) & 1ull);
         const bool open = live && !body && !accept;
