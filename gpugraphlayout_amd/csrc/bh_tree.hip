@@ -104,9 +104,15 @@ __global__ __launch_bounds__(256) void k_bbox_partials(const float2* __restrict_
 __global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, const float* __restrict__ mass, int n,
                                               const float4* __restrict__ part, DevState* st,
                                               float4* __restrict__ seg_root,
-                                              unsigned long long* __restrict__ keys,
-                                              unsigned int* __restrict__ vals, float4* __restrict__ pm) {
+                                              unsigned long long* __restrict__ k0, unsigned int* __restrict__ v0,
+                                              unsigned long long* __restrict__ k1, unsigned int* __restrict__ v1,
+                                              float4* __restrict__ pm, int first) {
     __shared__ float root[3];
+    // the buffer from which the sort passes that will run (8 less the skipped or unlaunched low
+    // ones) end in buffer 0
+    const bool odd = ((8 - max(st->sort_skip, first)) & 1) != 0;
+    unsigned long long* __restrict__ keys = odd ? k1 : k0;
+    unsigned int* __restrict__ vals = odd ? v1 : v0;
     reduce_bbox(part, st->nbbox, root);
     const float rootx = root[0], rooty = root[1], radius = root[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -771,11 +777,11 @@ void launch_bbox_partials(const Bufs& b, hipStream_t s) {
     k_bbox_partials<<<displacement_blocks(b), 256, 0, s>>>(b.pos, b.n, b.bbox_part, b.st);
 }
 
-void launch_keys(const Bufs& b, hipStream_t s) {
+void launch_keys(const Bufs& b, int first, hipStream_t s) {
     int blocks = (b.n + 255) / 256;
     if (blocks > b.grid_cap) blocks = b.grid_cap;
     k_keys<<<blocks, 256, 0, s>>>(b.pos, b.mass, b.n, b.bbox_part, b.st, b.seg_root, b.sort.keys[0], b.sort.vals[0],
-                                  b.pm);
+                                  b.sort.keys[1], b.sort.vals[1], b.pm, first);
 }
 
 void launch_tree(Bufs& b, hipStream_t s) {

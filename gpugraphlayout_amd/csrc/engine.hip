@@ -193,7 +193,7 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
             launch_seg_bbox(b, s);
             launch_seg_keys(b, s);
         } else {
-            launch_keys(b, s);
+            launch_keys(b, h->sort_first, s);
         }
     }
     mark(4, s);
@@ -204,8 +204,10 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
         } else {
             // the previous tree's depth bounds the key bits the order needs (skipped low passes
             // are made up for by the fix-up, so the order is the full sort's whatever the depth)
-            radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st, h->sort_first);
-            launch_sort_fixup(b.sort, b.n, b.st, s, h->sort_first);
+            // a single graph's keys start where the passes that run end in buffer 0
+            const bool lead = b.nseg == 1;
+            radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st, h->sort_first, lead);
+            launch_sort_fixup(b.sort, b.n, b.st, s, h->sort_first, lead);
             if (b.nseg > 1) launch_seg_graph_sort(b, s);
         }
     }

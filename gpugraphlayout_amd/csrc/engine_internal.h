@@ -212,12 +212,15 @@ constexpr int kTreeTile = kTreeThreads * kTreeItems;
 // launched: the pairs end up sorted by the key's top bits only, possibly in buffer 1, and
 // launch_sort_fixup (given the same `first`) then orders each run of equal top bits by the
 // whole key (and value), which gives the full sort's order, in buffer 0.
+// lead: the keys were written to the buffer from which the passes that run end in buffer 0
+// (key_buffer); otherwise they start in buffer 0.
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s,
-                      const DevState* skip = nullptr, int first = 0);
+                      const DevState* skip = nullptr, int first = 0, bool lead = false);
 // After radix_sort_pairs with a device skip: orders the runs of pairs that agree in every
 // sorted bit by (whole key, value) and leaves the whole result in keys[0]/vals[0] (the passes
 // that ran may have ended in the other buffer).
-void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first = 0);
+void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first = 0,
+                       bool lead = false);
 // Passes of the path-key sort below which the fix-up finishes the order: the smallest p whose
 // 4p-level cells of the previous tree hold at most this many bodies.
 constexpr int kSortRunMax = 16;
@@ -231,7 +234,9 @@ constexpr int kSegSortMax = 32768;
 void launch_attraction(const Bufs& b, const Params& p, hipStream_t s);
 void launch_col_rank(const Bufs& b, hipStream_t s);
 void launch_bbox_partials(const Bufs& b, hipStream_t s);
-void launch_keys(const Bufs& b, hipStream_t s);
+// first: the lowest path-key sort pass the host launches; the keys go to the buffer from which
+// the passes that will run end in buffer 0.
+void launch_keys(const Bufs& b, int first, hipStream_t s);
 void launch_tree(Bufs& b, hipStream_t s);              // after the sort: cells + summaries
 // split: the walk writes the repulsion alone (F_ga is not read), and launch_combine, after
 // both the walk and the attraction, forms F and the swing/traction sums.

@@ -32,19 +32,26 @@ __device__ __forceinline__ int passes_below(const DevState* __restrict__ st, int
     return max(st != nullptr ? st->sort_skip : 0, first);
 }
 
-// The buffer a pass reads (it writes the other one): the first pass that runs reads buffer 0,
-// and every pass that runs flips it (skipped passes move nothing).
-__device__ __forceinline__ int pass_src(const DevState* __restrict__ st, int first, int pass) {
-    return (pass - passes_below(st, first)) & 1;
+// The buffer the first pass that runs reads: buffer 0, or with `lead` (the total number of
+// passes, when the keys were written where an odd count of running passes needs them) the one
+// that makes the last pass end in buffer 0.
+__device__ __forceinline__ int start_buffer(const DevState* __restrict__ st, int first, int lead) {
+    return lead ? ((lead - passes_below(st, first)) & 1) : 0;
+}
+
+// The buffer a pass reads (it writes the other one): every pass that runs flips it (skipped
+// passes move nothing).
+__device__ __forceinline__ int pass_src(const DevState* __restrict__ st, int first, int lead, int pass) {
+    return (start_buffer(st, first, lead) + pass - passes_below(st, first)) & 1;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_upsweep(const unsigned long long* __restrict__ k0,
                                                           const unsigned long long* __restrict__ k1,
                                                           int n, int shift, unsigned int* __restrict__ hist,
                                                           int tiles, const DevState* __restrict__ st, int first,
-                                                          int pass) {
+                                                          int lead, int pass) {
     if (pass_skipped(st, pass)) return;
-    const unsigned long long* __restrict__ keys = pass_src(st, first, pass) ? k1 : k0;
+    const unsigned long long* __restrict__ keys = pass_src(st, first, lead, pass) ? k1 : k0;
     __shared__ unsigned int h[256];
     const int tid = threadIdx.x;
     h[tid] = 0;
@@ -225,9 +232,10 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(unsigned long long* __
                                                           unsigned int* __restrict__ v1, int n, int shift,
                                                           const unsigned int* __restrict__ hist, int tiles,
                                                           const unsigned int* __restrict__ rowtot,
-                                                          const DevState* __restrict__ st, int first, int pass) {
+                                                          const DevState* __restrict__ st, int first, int lead,
+                                                          int pass) {
     if (pass_skipped(st, pass)) return;
-    const int src = pass_src(st, first, pass);
+    const int src = pass_src(st, first, lead, pass);
     const unsigned long long* __restrict__ kin = src ? k1 : k0;
     const unsigned int* __restrict__ vin = src ? v1 : v0;
     unsigned long long* __restrict__ kout = src ? k0 : k1;
@@ -332,8 +340,11 @@ __global__ void k_sort_zero(unsigned int* __restrict__ digit_hist, int npasses) 
     for (int i = threadIdx.x; i < npasses * 256; i += blockDim.x) digit_hist[i] = 0u;
 }
 
-__global__ __launch_bounds__(256) void k_sort_hist(const unsigned long long* __restrict__ keys, int n, int begin_bit,
-                                                   int npasses, unsigned int* __restrict__ digit_hist) {
+__global__ __launch_bounds__(256) void k_sort_hist(const unsigned long long* __restrict__ k0,
+                                                   const unsigned long long* __restrict__ k1, int n, int begin_bit,
+                                                   int npasses, unsigned int* __restrict__ digit_hist,
+                                                   const DevState* __restrict__ st, int first, int lead) {
+    const unsigned long long* __restrict__ keys = start_buffer(st, first, lead) ? k1 : k0;
     __shared__ unsigned int h[8][256];
     for (int p = 0; p < npasses; ++p) h[p][threadIdx.x] = 0u;
     __syncthreads();
@@ -356,9 +367,9 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(unsigned long long* _
                                                            unsigned int* __restrict__ status,
                                                            unsigned long long* __restrict__ ticket, int tiles,
                                                            int* __restrict__ err, const DevState* __restrict__ st,
-                                                           int first, int pass) {
+                                                           int first, int lead, int pass) {
     if (pass_skipped(st, pass)) return;   // takes no ticket: the epochs of later passes stay distinct
-    const int src = pass_src(st, first, pass);
+    const int src = pass_src(st, first, lead, pass);
     const unsigned long long* __restrict__ kin = src ? k1 : k0;
     const unsigned int* __restrict__ vin = src ? v1 : v0;
     unsigned long long* __restrict__ kout = src ? k0 : k1;
@@ -470,10 +481,11 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
                                                     unsigned int* __restrict__ v0,
                                                     const unsigned long long* __restrict__ k1,
                                                     const unsigned int* __restrict__ v1, int n,
-                                                    const DevState* __restrict__ st, int first) {
+                                                    const DevState* __restrict__ st, int first, int lead) {
     const int skip = passes_below(st, first);
     if (skip == 0) return;                       // all 8 passes ran: sorted, ties by value, in buffer 0
-    const bool moved = ((8 - skip) & 1) != 0;    // the passes that ran ended in buffer 1
+    // the passes that ran ended in buffer 1 (unless the keys started where they end in buffer 0)
+    const bool moved = !lead && ((8 - skip) & 1) != 0;
     const unsigned long long* __restrict__ ks = moved ? k1 : k0;
     const unsigned int* __restrict__ vs = moved ? v1 : v0;
     const unsigned long long top = ~0ull << (8 * skip);
@@ -509,11 +521,12 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
 
 }  // namespace
 
-void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first) {
+void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first, bool lead) {
     if (n < 1) return;
     int blocks = (n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
-    k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], n, st, first);
+    k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], n, st, first,
+                                        lead ? 8 : 0);
 }
 
 void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s) {
@@ -522,10 +535,11 @@ void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* orde
 }
 
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s, const DevState* skip,
-                      int first) {
+                      int first, bool lead_in) {
     if (n <= 1) return;
     const int tiles = (n + kSortTile - 1) / kSortTile;
     const int npasses = (end_bit - begin_bit + 7) / 8;
+    const int lead = lead_in && skip ? npasses : 0;
     // Every pass picks its buffers on the device (pass_src): with a device skip the passes that
     // run may end in buffer 1, and launch_sort_fixup moves the result to buffer 0.
     unsigned long long *k0 = buf.keys[0], *k1 = buf.keys[1];
@@ -536,17 +550,17 @@ void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipSt
         k_sort_zero<<<1, 256, 0, s>>>(buf.digit_hist, npasses);
         int hb = (n + 255) / 256;
         if (hb > 256) hb = 256;
-        k_sort_hist<<<hb, 256, 0, s>>>(k0, n, begin_bit, npasses, buf.digit_hist);
+        k_sort_hist<<<hb, 256, 0, s>>>(k0, k1, n, begin_bit, npasses, buf.digit_hist, skip, first, lead);
         for (int p = first; p < npasses; ++p)
             k_onesweep<<<tiles, kSortThreads, 0, s>>>(k0, v0, k1, v1, n, begin_bit + 8 * p, buf.digit_hist + 256 * p,
-                                                      buf.status, buf.ticket, tiles, buf.err, skip, first, p);
+                                                      buf.status, buf.ticket, tiles, buf.err, skip, first, lead, p);
     } else {
         for (int p = first; p < npasses; ++p) {
             const int shift = begin_bit + 8 * p;
-            k_upsweep<<<tiles, kSortThreads, 0, s>>>(k0, k1, n, shift, buf.hist, tiles, skip, first, p);
+            k_upsweep<<<tiles, kSortThreads, 0, s>>>(k0, k1, n, shift, buf.hist, tiles, skip, first, lead, p);
             k_rowscan<<<256, 256, 0, s>>>(buf.hist, tiles, buf.rowtot, skip, p);
             k_scatter<<<tiles, kSortThreads, 0, s>>>(k0, v0, k1, v1, n, shift, buf.hist, tiles, buf.rowtot, skip, first,
-                                                     p);
+                                                     lead, p);
         }
     }
     if (!skip && ((npasses - first) & 1)) {
