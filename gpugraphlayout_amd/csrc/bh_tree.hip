@@ -421,7 +421,6 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum
         st->level_count[tid] = 0;
         st->level_cursor[tid] = 0;
     }
-    if (tid < 8) st->runmax[tid] = 0;
 }
 
 // S (exclusive prefix of opened cells), body records and brec.
@@ -515,9 +514,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
                                                         unsigned char* __restrict__ big_lvl) {
     __shared__ int hist[kMaxLevels + 1];
     __shared__ int lmax;
-    __shared__ int rmax[8];          // largest cell at level 4, 8, ..., 32 seen by the block
     if (threadIdx.x <= kMaxLevels) hist[threadIdx.x] = 0;
-    if (threadIdx.x < 8) rmax[threadIdx.x] = 0;
     if (threadIdx.x == 0) lmax = 0;
     __syncthreads();
     const bool ok = st->err == 0;
@@ -573,7 +570,6 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
         }
         rcount[rc] = b - i + 1;
         cell_rec[ord] = rc;       // the cell list of k_kids (and of the summary launches)
-        if (L > 0 && (L & 3) == 0 && L <= 32) atomicMax(&rmax[(L >> 2) - 1], b - i + 1);
         if (!pmass) {
             // the per-level cell lists of the summary launches (not needed for integer masses)
             cell_lvl[ord] = (unsigned char)L;
@@ -585,10 +581,6 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
     __syncthreads();
     if (!pmass && threadIdx.x <= kMaxLevels && hist[threadIdx.x] != 0)
         atomicAdd(&st->level_count[threadIdx.x], hist[threadIdx.x]);
-    // as for max_level: only blocks above the value seen so far touch the shared words
-    if (threadIdx.x < 8 && rmax[threadIdx.x] >
-                               __hip_atomic_load(&st->runmax[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMax(&st->runmax[threadIdx.x], rmax[threadIdx.x]);
     // every block would hit this one word: only blocks above the value seen so far update it
     // (the maximum only grows, so skipping on a stale value is still correct)
     if (threadIdx.x == 0 && lmax > 0 &&
@@ -599,14 +591,21 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
 // Children of every cell record, for the walk: a cell's subtree is the record run
 // [r, r + skip); its children are r + 1 and then each next record past the previous child's
 // subtree (a body spans one record), at most four. Unused slots hold -1.
-__global__ __launch_bounds__(256) void k_kids(const float4* __restrict__ rec, const int* __restrict__ S, int n,
+__global__ __launch_bounds__(256) void k_kids(const float4* __restrict__ rec, const int* __restrict__ rcount,
+                                              const int* __restrict__ S, int n,
                                               const int* __restrict__ cell_rec, int4* __restrict__ kids,
-                                              const DevState* __restrict__ st) {
-    if (st->err != 0) return;
+                                              int* __restrict__ kids_part, const DevState* __restrict__ st) {
+    __shared__ int rmax[8];          // largest cell at level 4, 8, ..., 32 seen by the block
+    if (st->err != 0) return;        // uniform
+    if (threadIdx.x < 8) rmax[threadIdx.x] = 0;
+    __syncthreads();
     const int ncells = S[n];
     for (int ord = blockIdx.x * 256 + threadIdx.x; ord < ncells; ord += gridDim.x * 256) {
         const int r = cell_rec[ord];
         const unsigned int info = __float_as_uint(rec[r].w);
+        // the level bits hold L - 1 (L >= 1): levels 4, 8, ..., 32 are 3, 7, ..., 31
+        const unsigned int lm1 = info & 31u;
+        if ((lm1 & 3u) == 3u) atomicMax(&rmax[lm1 >> 2], rcount[r]);
         const int end = r + (int)(info >> kInfoLevelBits);
         int k[4] = {-1, -1, -1, -1};
         int c = r + 1;
@@ -620,6 +619,10 @@ __global__ __launch_bounds__(256) void k_kids(const float4* __restrict__ rec, co
         }
         kids[r] = make_int4(k[0], k[1], k[2], k[3]);
     }
+    __syncthreads();
+    // per-block maxima, reduced by the speed step (no word shared by the blocks: thousands of
+    // device-scope reads or atomics on one line cost more than the kernel itself)
+    if (threadIdx.x < 8) kids_part[blockIdx.x * 8 + threadIdx.x] = rmax[threadIdx.x];
 }
 
 // Bucket cells by level (order inside a level is irrelevant).
@@ -802,12 +805,8 @@ void launch_tree(Bufs& b, hipStream_t s) {
     k_nodes<<<node_blocks, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, sg, n, b.S, b.rec, b.rcount, b.cell_rec,
                                                  b.cell_lvl, b.rec_begin, b.st, pm, b.seg_root, b.big_rc,
                                                  b.big_lvl);
-    {
-        // every record's info word is in place (cells: k_nodes, bodies: k_prefix)
-        int kb = (2 * n + 255) / 256;
-        if (kb > 4096) kb = 4096;
-        k_kids<<<kb, 256, 0, s>>>(b.rec, b.S, n, b.cell_rec, b.kids, b.st);
-    }
+    // every record's info word is in place (cells: k_nodes, bodies: k_prefix)
+    k_kids<<<b.kids_blocks, 256, 0, s>>>(b.rec, b.rcount, b.S, n, b.cell_rec, b.kids, b.kids_part, b.st);
     if (b.int_mass) {
         k_big<<<1, 256, 0, s>>>(b.rec, b.big_rc, b.big_lvl, b.st);
         return;

@@ -551,6 +551,8 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.S, (size_t)n + 1);
     GFA2_ALLOC(b.rec, (size_t)cap);
     GFA2_ALLOC(b.kids, (size_t)cap);
+    b.kids_blocks = std::max(1, std::min((n + 1023) / 1024, 1024));
+    GFA2_ALLOC(b.kids_part, (size_t)b.kids_blocks * 8);
     GFA2_ALLOC(b.brec, n);
     GFA2_ALLOC(b.sbody, n);
     GFA2_ALLOC(b.pm, n);
@@ -640,6 +642,7 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     }
     GFA2_TRY(hipMemcpyAsync(b.node_off, node_off, sizeof(int) * (nseg + 1), hipMemcpyHostToDevice, h->stream));
     GFA2_TRY(hipMemsetAsync(b.pos, 0, sizeof(float2) * n, h->stream));
+    GFA2_TRY(hipMemsetAsync(b.kids_part, 0, sizeof(int) * 8 * (size_t)b.kids_blocks, h->stream));
     GFA2_TRY(hipMemsetAsync(b.rec_begin, 0, sizeof(int) * (nseg + 1), h->stream));
     h->edges = nnz / 2;
     h->has_graph = true;

@@ -50,7 +50,7 @@ struct DevState {
     int nbbox;          // number of valid bounding-box partials
     int nbig;           // heavy cells listed for the in-order summary
     int sort_skip;      // low 8-bit digit passes the next path-key sort skips (0: a full sort)
-    int runmax[8];      // largest body count of a cell at level 4, 8, ..., 32 (this tree)
+    int runmax[8];      // largest body count of a cell at level 4, 8, ..., 32 (the last tree)
     int level_count[kMaxLevels + 1];
     int level_offset[kMaxLevels + 2];
     int level_cursor[kMaxLevels + 1];
@@ -144,6 +144,8 @@ struct Bufs {
     int* S;
     float4* rec;
     int4* kids;             // per cell record: the record indices of its 1-4 children (-1: none)
+    int* kids_part;         // [kids_blocks][8]: per k_kids block, the largest level-4p cell
+    int kids_blocks = 1;
     int* brec;
     float4* sbody;          // {x, y, mass, 0} of the body at sorted position k (node k in exact mode)
     float4* pm;             // {x, y, mass, 0} of node v, written with the keys (k_lcp gathers it)

@@ -583,12 +583,24 @@ constexpr int kSpeedThreads = 1024;
 __global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restrict__ part,
                                                          const int* __restrict__ grp_first,
                                                          const int* __restrict__ node_off, SegState* __restrict__ seg,
-                                                         DevState* st) {
+                                                         DevState* st, const int* __restrict__ kids_part,
+                                                         int kids_blocks) {
     __shared__ double rs[kSpeedThreads], rt[kSpeedThreads];
     __shared__ unsigned long long ri[kSpeedThreads];
+    __shared__ int rm[8];
     if (st->err != 0) return;
     const int g = blockIdx.x;
     const int tid = threadIdx.x;
+    if (g == 0) {
+        // the tree's largest cell at level 4, 8, ..., 32, from the child-list kernel's blocks
+        if (tid < 8) rm[tid] = 0;
+        __syncthreads();
+        // thread t reads level slot t & 7 of every 128th block (the stride keeps the slot)
+        int m = 0;
+        for (int q = tid; q < kids_blocks * 8; q += kSpeedThreads) m = max(m, kids_part[q]);
+        if (m > 0) atomicMax(&rm[tid & 7], m);
+        __syncthreads();
+    }
     const int g0 = grp_first[g], g1 = grp_first[g + 1];
     double s = 0.0, t = 0.0;
     unsigned long long it = 0;
@@ -664,8 +676,9 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restr
             // orders; take the fewest passes whose cells (in this tree) hold at most
             // kSortRunMax bodies each
             int p = 1;
-            while (p < 8 && st->runmax[p - 1] > kSortRunMax) ++p;
+            while (p < 8 && rm[p - 1] > kSortRunMax) ++p;
             st->sort_skip = 8 - p;
+            for (int q = 0; q < 8; ++q) st->runmax[q] = rm[q];
         }
     }
     // per graph (one block each): no word shared by the blocks of a batch
@@ -902,7 +915,8 @@ void launch_combine(const Bufs& b, const Params& p, hipStream_t s) {
 }
 
 void launch_speed(const Bufs& b, hipStream_t s) {
-    k_speed<<<b.nseg, kSpeedThreads, 0, s>>>(b.grp_part, b.grp_first, b.node_off, b.seg_st, b.st);
+    k_speed<<<b.nseg, kSpeedThreads, 0, s>>>(b.grp_part, b.grp_first, b.node_off, b.seg_st, b.st, b.kids_part,
+                                              b.kids_blocks);
 }
 
 void launch_displacement(const Bufs& b, hipStream_t s) {
