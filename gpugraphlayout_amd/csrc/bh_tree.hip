@@ -339,6 +339,43 @@ __global__ __launch_bounds__(kTreeThreads) void k_lcp(const unsigned long long* 
     }
 }
 
+// Two-level scan of the tile totals, for more tiles than one block scans quickly: per chunk
+// of kTileChunk tiles its totals (up), then one block scans the chunk totals (k_scan_blocks,
+// which also sets the tree's totals), then every chunk scans its tiles from its offset (down).
+// The sums are integers, so the result is the one-level scan's.
+__global__ __launch_bounds__(kTileChunk) void k_scan_up(const int* __restrict__ blocksum,
+                                                        const Moment* __restrict__ bmass, int nb,
+                                                        int* __restrict__ chunk_sum,
+                                                        Moment* __restrict__ chunk_mass) {
+    const int t = blockIdx.x * kTileChunk + threadIdx.x;
+    int total;
+    (void)block_exclusive_scan(t < nb ? blocksum[t] : 0, &total);
+    if (threadIdx.x == 0) chunk_sum[blockIdx.x] = total;
+    if (bmass) {
+        Moment mt;
+        (void)block_exclusive_scan_mo<kTileChunk>(t < nb ? bmass[t] : Moment{0ull, 0ull, 0ull, 0ull, 0ull}, &mt);
+        if (threadIdx.x == 0) chunk_mass[blockIdx.x] = mt;
+    }
+}
+
+__global__ __launch_bounds__(kTileChunk) void k_scan_down(int* __restrict__ blocksum, Moment* __restrict__ bmass,
+                                                          int nb, const int* __restrict__ chunk_sum,
+                                                          const Moment* __restrict__ chunk_mass) {
+    const int t = blockIdx.x * kTileChunk + threadIdx.x;
+    int total;
+    const int off = block_exclusive_scan(t < nb ? blocksum[t] : 0, &total) + chunk_sum[blockIdx.x];
+    Moment moff{0ull, 0ull, 0ull, 0ull, 0ull};
+    if (bmass) {
+        Moment mt;
+        moff = mo_add(block_exclusive_scan_mo<kTileChunk>(t < nb ? bmass[t] : Moment{0ull, 0ull, 0ull, 0ull, 0ull}, &mt),
+                      chunk_mass[blockIdx.x]);
+    }
+    if (t < nb) {
+        blocksum[t] = off;
+        if (bmass) bmass[t] = moff;
+    }
+}
+
 // Single block: exclusive scan of block sums (in place), cell total, capacity check,
 // per-iteration counters reset.
 __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum, int nb, int n,
@@ -795,7 +832,15 @@ void launch_tree(Bufs& b, hipStream_t s) {
     Moment* pm = b.int_mass ? b.pmass : nullptr;
     k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum, b.sort.vals[0], b.pm,
                                          b.seg_root, bm, b.sbody);
-    k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st, bm);
+    if (tiles > 8 * kTileChunk) {
+        const int chunks = (tiles + kTileChunk - 1) / kTileChunk;
+        Moment* cm = b.int_mass ? b.chunk_mass : nullptr;
+        k_scan_up<<<chunks, kTileChunk, 0, s>>>(b.blocksum, bm, tiles, b.chunk_sum, cm);
+        k_scan_blocks<<<1, 1024, 0, s>>>(b.chunk_sum, chunks, n, b.cap_records, b.rec_begin + b.nseg, b.st, cm);
+        k_scan_down<<<chunks, kTileChunk, 0, s>>>(b.blocksum, bm, tiles, b.chunk_sum, cm);
+    } else {
+        k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st, bm);
+    }
     const int max_cells = b.cap_records - n;
     k_prefix<<<tiles, kTreeThreads, 0, s>>>(b.lcp, sg, n, b.blocksum, b.S, b.sbody, b.rec, b.brec, b.st, b.seg_root,
                                             bm, pm, b.cell_rec, max_cells);

@@ -199,8 +199,10 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
     mark(4, s);
     if (!h->p.exact) {
         if (b.nseg > 1 && b.max_seg <= kSegSortMax) {
-            // every graph fits one block's sort: by (graph, key) in one launch
-            segmented_sort_pairs(b.sort, b.node_off, b.seg_order, b.nseg, s);
+            // every graph fits one block's sort: by (graph, key) in one launch, the low passes
+            // skipped as for one graph; the result is in buffer 0 and runs stop at graphs
+            segmented_sort_pairs(b.sort, b.node_off, b.seg_order, b.nseg, s, b.st);
+            launch_sort_fixup(b.sort, b.n, b.st, s, 0, true, b.seg_of);
         } else {
             // the previous tree's depth bounds the key bits the order needs (skipped low passes
             // are made up for by the fix-up, so the order is the full sort's whatever the depth)
@@ -548,10 +550,12 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_TRY(hipMemsetAsync(b.sort.ticket, 0, sizeof(unsigned long long), h->stream));
     GFA2_ALLOC(b.lcp, n);
     GFA2_ALLOC(b.blocksum, tree_tiles);
+    GFA2_ALLOC(b.chunk_sum, (tree_tiles + kTileChunk - 1) / kTileChunk);
     GFA2_ALLOC(b.S, (size_t)n + 1);
     GFA2_ALLOC(b.rec, (size_t)cap);
     GFA2_ALLOC(b.kids, (size_t)cap);
-    b.kids_blocks = std::max(1, std::min((n + 1023) / 1024, 1024));
+    // enough threads for the cells' dependent child hops (about one cell per thread)
+    b.kids_blocks = std::max(1, std::min((n + 255) / 256, 16384));
     GFA2_ALLOC(b.kids_part, (size_t)b.kids_blocks * 8);
     GFA2_ALLOC(b.brec, n);
     GFA2_ALLOC(b.sbody, n);
@@ -610,6 +614,7 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     if (b.int_mass) {
         GFA2_ALLOC(b.pmass, (size_t)n + 1);
         GFA2_ALLOC(b.bmass, tree_tiles);
+        GFA2_ALLOC(b.chunk_mass, (tree_tiles + kTileChunk - 1) / kTileChunk);
         GFA2_ALLOC(b.big_rc, kMaxBig);
         GFA2_ALLOC(b.big_lvl, kMaxBig);
     }

@@ -183,6 +183,8 @@ struct Bufs {
     bool int_mass = false;
     Moment* pmass = nullptr;          // n + 1
     Moment* bmass = nullptr;          // per tree tile
+    int* chunk_sum = nullptr;         // per 256 tree tiles (the two-level tile scan)
+    Moment* chunk_mass = nullptr;
     int* big_rc = nullptr;            // kMaxBig
     unsigned char* big_lvl = nullptr; // kMaxBig
 };
@@ -207,6 +209,7 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kTreeThreads = 256;
 constexpr int kTreeItems = 2;
 constexpr int kTreeTile = kTreeThreads * kTreeItems;
+constexpr int kTileChunk = 256;       // tree tiles per block of the two-level tile scan
 
 // Radix sort of (key, value) pairs on 64-bit keys, ascending and stable. Sorts
 // buf.keys[0]/vals[0] in place (via buf.keys[1]/vals[1]). With skip (path keys only, begin_bit
@@ -221,15 +224,20 @@ void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipSt
 // After radix_sort_pairs with a device skip: orders the runs of pairs that agree in every
 // sorted bit by (whole key, value) and leaves the whole result in keys[0]/vals[0] (the passes
 // that ran may have ended in the other buffer).
+// seg: the graph of each sorted position when every graph was sorted in its own range (runs
+// then stop at graph boundaries); null for one graph or a global sort.
 void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first = 0,
-                       bool lead = false);
+                       bool lead = false, const int* seg = nullptr);
 // Passes of the path-key sort below which the fix-up finishes the order: the smallest p whose
 // 4p-level cells of the previous tree hold at most this many bodies.
 constexpr int kSortRunMax = 16;
 // Sorts each segment [node_off[g], node_off[g+1]) of buf.keys[0]/vals[0] in place by the full
 // 64-bit key, stable, one block per segment (taken in `order`); for batches whose graphs all
 // have at most kSegSortMax nodes.
-void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s);
+// skip: the state whose sort_skip low passes are skipped (then launch_sort_fixup with the
+// graph of each position, and lead, finishes the order); null: all 8 passes.
+void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s,
+                          const DevState* skip = nullptr);
 constexpr int kSegSortMax = 32768;
 
 // Launch groups of one iteration, in order.

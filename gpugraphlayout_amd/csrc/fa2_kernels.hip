@@ -595,9 +595,19 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restr
         // the tree's largest cell at level 4, 8, ..., 32, from the child-list kernel's blocks
         if (tid < 8) rm[tid] = 0;
         __syncthreads();
-        // thread t reads level slot t & 7 of every 128th block (the stride keeps the slot)
+        // thread t reads level slot t & 7 of every 128th block (the stride keeps the slot),
+        // eight loads in flight
         int m = 0;
-        for (int q = tid; q < kids_blocks * 8; q += kSpeedThreads) m = max(m, kids_part[q]);
+        const int nq = kids_blocks * 8;
+        int q = tid;
+        for (; q + 7 * kSpeedThreads < nq; q += 8 * kSpeedThreads) {
+            int v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = kids_part[q + u * kSpeedThreads];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) m = max(m, v[u]);
+        }
+        for (; q < nq; q += kSpeedThreads) m = max(m, kids_part[q]);
         if (m > 0) atomicMax(&rm[tid & 7], m);
         __syncthreads();
     }

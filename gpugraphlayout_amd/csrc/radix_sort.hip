@@ -271,12 +271,16 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_sort(unsigned long long* _
                                                            unsigned long long* __restrict__ k1,
                                                            unsigned int* __restrict__ v1,
                                                            const int* __restrict__ node_off,
-                                                           const int* __restrict__ order) {
+                                                           const int* __restrict__ order,
+                                                           const DevState* __restrict__ st) {
     __shared__ TileLds L;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int g = order[blockIdx.x];
+    // the low passes the previous trees' cells let the sort skip (k_sort_fixup, told the graph
+    // of each position, orders what they would have ordered)
+    const int p0 = st ? st->sort_skip : 0;
     const int a = node_off[g];
     const int m = node_off[g + 1] - a;
     // the 8 digit histograms (8 x 256 words), counted in the not yet used tile staging area when
@@ -298,12 +302,17 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_sort(unsigned long long* _
 #pragma unroll
     for (int p = 0; p < 8; ++p) hc[p] = hist[p * 256 + tid];
     __syncthreads();                             // the staging area is the tiles' again
+    for (int p = 0; p < p0; ++p) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) hc[q] = hc[q + 1];
+    }
     const int tiles = (m + kSortTile - 1) / kSortTile;
-    for (int p = 0; p < 8; ++p) {
-        const unsigned long long* kin = ((p & 1) ? k1 : k0) + a;
-        const unsigned int* vin = ((p & 1) ? v1 : v0) + a;
-        unsigned long long* kout = ((p & 1) ? k0 : k1) + a;
-        unsigned int* vout = ((p & 1) ? v0 : v1) + a;
+    for (int p = p0; p < 8; ++p) {
+        const bool from1 = ((p - p0) & 1) != 0;
+        const unsigned long long* kin = (from1 ? k1 : k0) + a;
+        const unsigned int* vin = (from1 ? v1 : v0) + a;
+        unsigned long long* kout = (from1 ? k0 : k1) + a;
+        unsigned int* vout = (from1 ? v0 : v1) + a;
         // this thread's digit: its first destination in the segment, advanced tile by tile
         unsigned int dstart = block_exscan256(hc[0], L.sh[0]);
 #pragma unroll
@@ -326,6 +335,13 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_sort(unsigned long long* _
         }
         // this pass's writes are read by the next one (one block: workgroup-scope visibility)
         __syncthreads();
+    }
+    if (((8 - p0) & 1) != 0) {
+        // an odd number of passes ran: the range is sorted in the second buffer
+        for (int e = tid; e < m; e += kSortThreads) {
+            k0[a + e] = k1[a + e];
+            v0[a + e] = v1[a + e];
+        }
     }
 }
 
@@ -481,7 +497,8 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
                                                     unsigned int* __restrict__ v0,
                                                     const unsigned long long* __restrict__ k1,
                                                     const unsigned int* __restrict__ v1, int n,
-                                                    const DevState* __restrict__ st, int first, int lead) {
+                                                    const DevState* __restrict__ st, int first, int lead,
+                                                    const int* __restrict__ seg) {
     const int skip = passes_below(st, first);
     if (skip == 0) return;                       // all 8 passes ran: sorted, ties by value, in buffer 0
     // the passes that ran ended in buffer 1 (unless the keys started where they end in buffer 0)
@@ -489,11 +506,13 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
     const unsigned long long* __restrict__ ks = moved ? k1 : k0;
     const unsigned int* __restrict__ vs = moved ? v1 : v0;
     const unsigned long long top = ~0ull << (8 * skip);
+    // seg (sorted position -> graph, each graph sorted in its own range): runs stop at graphs
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const unsigned long long ki = ks[i] & top;
-        if (i > 0 && (ks[i - 1] & top) == ki) continue;     // not the first of its run
+        const int gi = seg ? seg[i] : 0;
+        if (i > 0 && (ks[i - 1] & top) == ki && (!seg || seg[i - 1] == gi)) continue;   // not a run's first
         int j = i + 1;
-        while (j < n && (ks[j] & top) == ki) ++j;
+        while (j < n && (ks[j] & top) == ki && (!seg || seg[j] == gi)) ++j;
         if (moved)
             for (int a = i; a < j; ++a) {
                 k0[a] = ks[a];
@@ -521,17 +540,20 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
 
 }  // namespace
 
-void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first, bool lead) {
+void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first, bool lead,
+                       const int* seg) {
     if (n < 1) return;
     int blocks = (n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
     k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], n, st, first,
-                                        lead ? 8 : 0);
+                                        lead ? 8 : 0, seg);
 }
 
-void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s) {
+void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s,
+                          const DevState* skip) {
     if (nseg > 0)
-        k_seg_sort<<<nseg, kSortThreads, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], node_off, order);
+        k_seg_sort<<<nseg, kSortThreads, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], node_off, order,
+                                                 skip);
 }
 
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s, const DevState* skip,
