@@ -531,11 +531,10 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
     if (i0 + kTreeItems >= n && i0 < n) S[n] = run;   // total cells
 }
 
-// Cell records (skip, level, count) and the cell list for level bucketing, one thread per
-// cell (grid-stride over the S[n] cells). cell_rec[ord] holds the cell's first sorted
-// position on entry (written by k_prefix); on the fractional-mass path it holds the cell's
-// record index on exit (with cell_lvl, for the per-level summaries), on the integer-mass
-// path it is left as it was (the summary comes from the moments). The cells opened at
+// Cell records (skip, level, count) and the cell list, one thread per cell (grid-stride over
+// the S[n] cells). cell_rec[ord] holds the cell's first sorted position on entry (written by
+// k_prefix) and the cell's record index on exit (the list k_kids walks; on the fractional-mass
+// path, with cell_lvl, also the per-level summaries' list). The cells opened at
 // position i have levels lprev+1 .. lcp_i in ord order; each finds its extent end by a
 // galloping search over the keys (sorted within a graph, so a cell's bodies are contiguous).
 // One thread per cell keeps the long searches of the top levels, which all start at a
