@@ -276,10 +276,14 @@ __device__ __forceinline__ int xcd_block() {
 // The epilogue adds gravity + attraction, writes the total force, and forms the group's
 // swing/traction sums (SpeedKernel, src/RPFA2Kernels.cu:111-161).
 // ---------------------------------------------------------------------------------------
-constexpr int kWalkStack = 512;     // stack entries (record indices) per wave
+// LDS per wave: 448 * 4 + (256 + 8) * 12 = 4,960 bytes, so that eight waves fit per SIMD
+// (eight 256-thread blocks of 19,840 bytes in the CU's 160 KiB); full 64-entry rounds need
+// top <= 128, far above the stacks the walks reach
+constexpr int kWalkStack = 448;     // stack entries (record indices) per wave
 constexpr int kWalkSlack = 128;     // the depth-first rounds' headroom: at most 3 per tree level
 constexpr int kWalkList = 256;      // accepted records buffered per wave
 static_assert(kWalkSlack >= 3 * (kMaxLevels + 1) + 4, "a depth-first descent must fit the headroom");
+static_assert(kWalkStack - kWalkSlack >= 64 * 3 + 128, "a full round must fit above a stack of 128");
 
 __device__ __forceinline__ float vote_d2(float px, float py, float cx, float cy, float epssq) {
     const float dx = px - cx, dy = py - cy;
@@ -346,6 +350,10 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     // the lanes whose bodies vote: the group's active bodies (for G = 32 in both half-waves,
     // which hold the same bodies; each half is read as its own 32-bit vote)
     const unsigned long long voters = __builtin_amdgcn_ballot_w64(bact);
+    // each half-wave's voters (G = 32), and the byte offset that selects a half-wave's own cell
+    const unsigned long long vlo = voters & __builtin_amdgcn_ballot_w64(lane < 32);
+    const unsigned long long vhi = voters & __builtin_amdgcn_ballot_w64(lane >= 32);
+    const int upper4 = (lane >> 5) << 2;    // 4 in the upper half-wave, else 0 (j2 >= j)
     const float two_r = seg_root[g].z * 2;
     const float base = two_r * two_r * itolsq;
     // the root's children start the walk (a graph of one body has no cell records)
@@ -375,6 +383,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         if (room < take) take = room > 1 ? room : 1;
         top -= take;
         const bool live = lane < take;
+        const unsigned long long livem = take >= 64 ? ~0ull : (1ull << take) - 1ull;
         // lanes past `take` repeat the last entry (in bounds) and take no part below
         const int r = stk[top + min(lane, take - 1)];
         const float4 c = rec[r];
@@ -382,16 +391,20 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
         const unsigned int info = __float_as_uint(c.w);
         const bool body = info == 0xffffffffu;
         const float thr = ldexpf(base, -2 * (int)(info & 31u)) + epssq;
-        const float ex = c.x < mnx ? mnx - c.x : (c.x > mxx ? c.x - mxx : 0.0f);
-        const float ey = c.y < mny ? mny - c.y : (c.y > mxy ? c.y - mxy : 0.0f);
-        const bool clear = fmaf(ex, ex, fmaf(ey, ey, epssq)) >= thr;    // every body's d2 >= thr
-        // the first, middle and last bodies' own d2: any below thr rejects
-        const bool pass3 = vote_d2(b0x, b0y, c.x, c.y, epssq) >= thr && vote_d2(bmx, bmy, c.x, c.y, epssq) >= thr &&
-                           vote_d2(blx, bly, c.x, c.y, epssq) >= thr;
+        // distance from the group's box (mnx <= mxx: at most one term is positive)
+        const float ex = fmaxf(fmaxf(mnx - c.x, c.x - mxx), 0.0f);
+        const float ey = fmaxf(fmaxf(mny - c.y, c.y - mxy), 0.0f);
+        // lane masks: bodies; cells every body clears (d2 >= thr); cells the first, middle and
+        // last bodies all clear (any of them below thr rejects)
+        const unsigned long long bodym = __builtin_amdgcn_ballot_w64(body);
+        const unsigned long long clearm = __builtin_amdgcn_ballot_w64(fmaf(ex, ex, fmaf(ey, ey, epssq)) >= thr);
+        const unsigned long long p3m = __builtin_amdgcn_ballot_w64(vote_d2(b0x, b0y, c.x, c.y, epssq) >= thr) &
+                                       __builtin_amdgcn_ballot_w64(vote_d2(bmx, bmy, c.x, c.y, epssq) >= thr) &
+                                       __builtin_amdgcn_ballot_w64(vote_d2(blx, bly, c.x, c.y, epssq) >= thr);
         // the remaining cells, voted on by the group's bodies (lanes = bodies); with G = 32 the
         // two half-waves hold the same bodies and vote on two cells at once
-        unsigned long long okm = __builtin_amdgcn_ballot_w64(live && (body || clear));
-        unsigned long long um = __builtin_amdgcn_ballot_w64(live && !body && !clear && pass3);
+        unsigned long long okm = livem & (bodym | clearm);
+        unsigned long long um = livem & p3m & ~(bodym | clearm);
         while (um) {
             const int j = __builtin_ctzll(um);
             um &= um - 1;
@@ -405,7 +418,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
             float cx, cy, tj;
             if constexpr (G == 32) {
                 // each half-wave fetches its own cell's values (the upper half: the second cell)
-                const int src = (lane >= 32 ? j2 : j) << 2;
+                const int src = (j << 2) + upper4 * (j2 - j);
                 cx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(c.x)));
                 cy = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(c.y)));
                 tj = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(thr)));
@@ -414,24 +427,23 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
                 cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.y), j));
                 tj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr), j));
             }
-            const unsigned long long fails =
-                __builtin_amdgcn_ballot_w64(!(vote_d2(px, py, cx, cy, epssq) >= tj)) & voters;
+            const unsigned long long fails = __builtin_amdgcn_ballot_w64(!(vote_d2(px, py, cx, cy, epssq) >= tj));
             if constexpr (G == 32) {
-                if ((unsigned int)fails == 0u) okm |= 1ull << j;
-                if ((unsigned int)(fails >> 32) == 0u) okm |= 1ull << j2;
+                if ((fails & vlo) == 0ull) okm |= 1ull << j;
+                if ((fails & vhi) == 0ull) okm |= 1ull << j2;
             } else {
-                if (fails == 0ull) okm |= 1ull << j;
+                if ((fails & voters) == 0ull) okm |= 1ull << j;
             }
         }
-        const bool accept = live && ((okm >> lane) & 1ull);
+        // opened cells push their 1-4 children, in four planes (every opened cell's first
+        // child, then every second child, ...); okm holds live lanes only
+        const unsigned long long am = okm;
+        const unsigned long long m0 = livem & ~bodym & ~okm;
+        const unsigned long long m1 = m0 & __builtin_amdgcn_ballot_w64(kd.y >= 0);
+        const unsigned long long m2 = m0 & __builtin_amdgcn_ballot_w64(kd.z >= 0);
+        const unsigned long long m3 = m0 & __builtin_amdgcn_ballot_w64(kd.w >= 0);
+        const bool accept = (okm >> lane) & 1ull;
         const bool open = live && !body && !accept;
-        // an opened cell pushes its 1-4 children, in four planes (every opened cell's first
-        // child, then every second child, ...)
-        const unsigned long long m0 = __builtin_amdgcn_ballot_w64(open);
-        const unsigned long long m1 = __builtin_amdgcn_ballot_w64(open && kd.y >= 0);
-        const unsigned long long m2 = __builtin_amdgcn_ballot_w64(open && kd.z >= 0);
-        const unsigned long long m3 = __builtin_amdgcn_ballot_w64(open && kd.w >= 0);
-        const unsigned long long am = __builtin_amdgcn_ballot_w64(accept);
         const int p1 = top + __popcll(m0), p2 = p1 + __popcll(m1), p3 = p2 + __popcll(m2);
         const int na = __popcll(am);
         if (open) {
