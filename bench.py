@@ -7,9 +7,11 @@ Workload (default): R-MAT over 2^20 ids, 10,000,000 distinct undirected non-loop
 of side 2*sqrt(V); weak gravity k_g = 1, k_r = 10, theta = 1. One step = one FA2
 iteration (tree + walk + attraction + gravity + speed + displacement), all inputs
 resident in HBM. W untimed warm-up iterations, then K timed ones, bracketed by a device
-synchronisation on both sides: eager launches with HIP events around every kernel group (the
-kernel times of the roofline), or for ba10k, whose iteration is bound by launch latency, K
-replays of the iteration captured as one hipGraph, after an equal eager pass with events.
+synchronisation on both sides: eager launches with HIP events around the force walk only (the
+kernel time of the roofline; every event costs a little), or for ba10k, whose iteration is
+bound by launch latency, K replays of the iteration captured as one hipGraph, after an equal
+eager pass with the walk's events. The per-group breakdown comes from a further K eager
+iterations with events around every kernel group.
 
 Also reported:
   roofline      the Barnes-Hut walk's (FP32 VALU: 12 flops per body-node interaction,
@@ -98,7 +100,8 @@ def cpu_baseline(g, pos, strong, k_g, k_r, iters, warm=1, keep=False):
 
 
 def timed_run(eng, args, steps):
-    """Autotune (optional), then `steps` timed iterations with per-group HIP events."""
+    """Autotune (optional), then `steps` timed eager iterations with HIP events around the
+    walk (its average time per launch, in ms, is returned with the region's wall time)."""
     tune_s = None
     if args.walk_block > 0:
         eng.set_launch_config(args.walk_block, args.blocks_per_cu, args.attraction_stream,
@@ -110,22 +113,36 @@ def timed_run(eng, args, steps):
         tune_s = time.perf_counter() - t0
         log(f"autotune {launch} in {tune_s:.2f}s")
     s0 = eng.stats()
-    eng.set_timing(True)
+    eng.set_timing("walk")
     eng.sync()
     t0 = time.perf_counter()
     eng.step(steps)
     eng.sync()
     dt = time.perf_counter() - t0
-    groups = eng.timing()
+    walk_ms = eng.timing()["walk"]
     eng.set_timing(False)
     s1 = eng.stats()
-    log(f"{steps / dt:.1f} it/s; groups(ms) {json.dumps({k: round(v, 4) for k, v in groups.items()})}")
-    return dt, groups, s0, s1, launch, tune_s
+    log(f"{steps / dt:.1f} it/s; walk {walk_ms:.4f} ms")
+    return dt, walk_ms, s0, s1, launch, tune_s
 
 
-def walk_roofline(groups, s0, s1, steps, V, split):
+def group_breakdown(eng, steps):
+    """`steps` more eager iterations with HIP events around every kernel group: the average ms
+    per iteration of each group, and the pass's own rate (the events slow it a little)."""
+    eng.set_timing(True)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.sync()
+    rate = steps / (time.perf_counter() - t0)
+    groups = eng.timing()
+    eng.set_timing(False)
+    log(f"groups(ms) {json.dumps({k: round(v, 4) for k, v in groups.items()})} at {rate:.1f} it/s")
+    return groups, rate
+
+
+def walk_roofline(walk_ms, s0, s1, steps, V, split):
     inter = (s1["interactions_total"] - s0["interactions_total"]) / steps
-    walk_ms = groups["walk"]
     tflops = 12.0 * inter / (walk_ms * 1e-3) / 1e12
     return {
         "bound": "valu",
@@ -309,15 +326,13 @@ def run_single(args):
     eng.step(args.warmup)
     eng.sync()
     log(f"warm-up done ({args.warmup} it), timing {steps} it")
-    dt, groups, s0, s1, launch, tune_s = timed_run(eng, args, steps)
+    dt, walk_ms, s0, s1, launch, tune_s = timed_run(eng, args, steps)
     value = steps / dt
-    events_value = value
-    timed = args.timed or ("replay" if args.workload == "ba10k" else "events")
+    eager_value = value
+    timed = args.timed or ("replay" if args.workload == "ba10k" else "eager")
 
     V, E = g.n, g.m
-    sb = streaming_bytes(V, E)
-    hbm = {k: {"bytes": b, "ms": groups[k], "GB/s": b / (groups[k] * 1e-3) / 1e9} for k, b in sb.items()}
-    roof = walk_roofline(groups, s0, s1, steps, V, launch["attraction_stream"] == 1)
+    roof = walk_roofline(walk_ms, s0, s1, steps, V, launch["attraction_stream"] == 1)
 
     replay = None
     replay_iters = steps if timed == "replay" else args.graph_replay
@@ -333,11 +348,14 @@ def run_single(args):
         replay = replay_iters / replay_dt
         eng.set_graph_capture(False)
         if timed == "replay":
-            # the timed region: K replayed iterations; the kernel groups and the roofline come
-            # from the K eager iterations with HIP events just before
+            # the timed region: K replayed iterations; the roofline comes from the K eager
+            # iterations with the walk's HIP events just before
             dt, value = replay_dt, replay
             log(f"hipGraph replay (timed region): {value:.1f} it/s")
     final = eng.stats()
+    groups, groups_rate = group_breakdown(eng, steps)
+    sb = streaming_bytes(V, E)
+    hbm = {k: {"bytes": b, "ms": groups[k], "GB/s": b / (groups[k] * 1e-3) / 1e9} for k, b in sb.items()}
 
     # layout quality: a fresh run of the workload's own iteration count from the same start, so
     # the GPU's and the CPU baseline's final layouts are comparable (ba10k runs the CPU path to
@@ -390,11 +408,14 @@ def run_single(args):
                            walk_block=launch["walk_block"], blocks_per_cu=launch["blocks_per_cu"],
                            attraction_stream=launch["attraction_stream"],
                            attraction_blocks_per_cu=launch["attraction_blocks_per_cu"],
-                           timed=("hipGraph replay, one launch per iteration; kernel groups and roofline from "
-                                  "the same number of eager iterations with HIP events just before"
-                                  if timed == "replay" else "eager launches with HIP events around every group")))
+                           timed=("hipGraph replay, one launch per iteration; the roofline from the same number "
+                                  "of eager iterations with HIP events around the walk just before"
+                                  if timed == "replay" else "eager launches with HIP events around the walk"),
+                           kernel_groups=("a further pass of the same number of eager iterations with HIP events "
+                                          "around every group")))
     out.update({
-        "eager_events_iterations_per_s": events_value,
+        "eager_iterations_per_s": eager_value,
+        "groups_pass_iterations_per_s": groups_rate,
         "roofline": roof,
         "cpu_baseline": cpu,
         "time_to_equal_stress": tte,
@@ -433,9 +454,10 @@ def run_batch(args):
     eng.step(args.warmup)
     eng.sync()
     log(f"warm-up done ({args.warmup} it), timing {steps} it")
-    dt, groups, s0, s1, launch, tune_s = timed_run(eng, args, steps)
+    dt, walk_ms, s0, s1, launch, tune_s = timed_run(eng, args, steps)
     value = steps / dt
     batch_final = eng.positions()
+    groups, groups_rate = group_breakdown(eng, steps)
     replay = None
     if args.graph_replay > 0:
         # the whole batch's iteration captured once as a hipGraph, then replayed
@@ -495,11 +517,12 @@ def run_batch(args):
            "attraction_blocks_per_cu": launch["attraction_blocks_per_cu"]}
     out = base_record(args, steps, dt, value, cfg)
     out.update({
-        "roofline": walk_roofline(groups, s0, s1, steps, V, launch["attraction_stream"] == 1),
+        "roofline": walk_roofline(walk_ms, s0, s1, steps, V, launch["attraction_stream"] == 1),
         "cpu_baseline": cpu,
         "graph_iterations_per_s": value * len(items),
         "nodes_edges_per_s": (V + E) * value,
         "kernel_groups_ms": groups,
+        "groups_pass_iterations_per_s": groups_rate,
         "sequential_estimate_s": seq_total,
         "sequential_sample_graphs": int(len(sample)),
         "sequential_sample_bit_identical": int(same),
@@ -533,10 +556,10 @@ def main():
                     help="with --walk-block: attraction grid per compute unit (0: one block per work item)")
     ap.add_argument("--graph-replay", type=int, default=100,
                     help="extra iterations replayed as one captured hipGraph per iteration (reported apart)")
-    ap.add_argument("--timed", choices=("events", "replay"), default=None,
-                    help="single graphs: the timed region as eager launches with HIP events (default) or as "
-                         "hipGraph replays (default for ba10k, whose iteration is launch-bound), kernel groups "
-                         "then from an equal eager pass")
+    ap.add_argument("--timed", choices=("eager", "replay"), default=None,
+                    help="single graphs: the timed region as eager launches with HIP events around the walk "
+                         "(default) or as hipGraph replays (default for ba10k, whose iteration is launch-bound), "
+                         "the walk's time then from an equal eager pass")
     ap.add_argument("--traffic", type=int, default=None,
                     help="1: measure the walk's HBM bytes with rocprofv3 counters in child runs (default: on for "
                          "rmat20)")

@@ -383,8 +383,11 @@ class Engine:
                          0.0)
         _check(lib().gfa2_set_launch_config(self.h, ctypes.byref(c)), "gfa2_set_launch_config")
 
-    def set_timing(self, on: bool = True):
-        _check(lib().gfa2_set_timing(self.h, int(bool(on))), "gfa2_set_timing")
+    def set_timing(self, on=True):
+        """True (or 1): events around every kernel group; "walk" (or 2): only around the
+        force walk, the least added work; False: off."""
+        mode = 2 if on == "walk" else int(on)
+        _check(lib().gfa2_set_timing(self.h, mode), "gfa2_set_timing")
 
     def timing(self) -> dict:
         names = (ctypes.c_char_p * 32)()

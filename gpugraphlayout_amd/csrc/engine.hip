@@ -26,6 +26,7 @@ constexpr int kGroups = 8;
 // Events 0 and 11 bracket the whole iteration.
 constexpr int kEvents = 12;
 constexpr int kSpan[kGroups][2] = {{1, 2}, {3, 4}, {4, 5}, {5, 6}, {7, 8}, {8, 9}, {9, 10}, {10, 11}};
+constexpr int kWalkGroup = 4;
 
 }  // namespace
 
@@ -46,7 +47,7 @@ struct gfa2_engine {
     float* ew_buf = nullptr;            // w^delta per CSR entry, once weights were given
     std::vector<void*> allocs;
     // timing
-    bool timing = false;
+    int timing = 0;                     // 0, GFA2_TIMING_GROUPS or GFA2_TIMING_WALK
     std::vector<hipEvent_t> events;     // pool, kEvents per pending iteration
     int pending = 0;                    // iterations with recorded, unread events
     double acc_ms[kGroups] = {0};
@@ -162,8 +163,10 @@ constexpr int kStressBlocks = 512;
 void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
     hipStream_t s = h->stream;
     Bufs& b = h->b;
+    // with GFA2_TIMING_WALK only the walk's pair is recorded (each mark costs a little)
     auto mark = [&](int i, hipStream_t on) {
-        if (ev) hipEventRecord(ev[i], on);
+        if (ev && (h->timing != GFA2_TIMING_WALK || i == kSpan[kWalkGroup][0] || i == kSpan[kWalkGroup][1]))
+            hipEventRecord(ev[i], on);
     };
     // Gravity + attraction reads only the positions, which nothing in the iteration writes
     // before the displacement, and writes only F_ga, which only the walk's epilogue reads. So it
@@ -240,6 +243,7 @@ int drain_timing(gfa2_engine* h) {
     for (int it = 0; it < h->pending; ++it) {
         hipEvent_t* ev = &h->events[(size_t)it * kEvents];
         for (int g = 0; g < kGroups; ++g) {
+            if (h->timing == GFA2_TIMING_WALK && g != kWalkGroup) continue;
             float ms = 0.0f;
             GFA2_TRY(hipEventElapsedTime(&ms, ev[kSpan[g][0]], ev[kSpan[g][1]]));
             h->acc_ms[g] += ms;
@@ -763,8 +767,11 @@ int gfa2_step(gfa2_handle h, int32_t iters) {
             const int now = std::min(left, kTimedBatch - h->pending);
             const size_t need = (size_t)(h->pending + now) * kEvents;
             while (h->events.size() < need) {
+                // timing only: no system-scope fence (its cache writeback would slow the
+                // kernels between the marks); the stream order and the kernels' own fences
+                // are what orders the work
                 hipEvent_t e;
-                GFA2_TRY(hipEventCreate(&e));
+                GFA2_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
                 h->events.push_back(e);
             }
             for (int i = 0; i < now; ++i) {
@@ -915,12 +922,12 @@ int gfa2_get_stats(gfa2_handle h, gfa2_stats* s) {
 }
 
 int gfa2_set_timing(gfa2_handle h, int32_t on) {
-    if (!h) return GFA2_ERR_INVALID_ARG;
+    if (!h || on < 0 || on > GFA2_TIMING_WALK) return GFA2_ERR_INVALID_ARG;
     if (h->pending) {
         int rc = drain_timing(h);
         if (rc != GFA2_OK) return rc;
     }
-    h->timing = on != 0;
+    h->timing = on;
     for (double& a : h->acc_ms) a = 0.0;
     h->timed_iters = 0;
     return GFA2_OK;
@@ -930,11 +937,17 @@ int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** nam
     if (!h) return GFA2_ERR_INVALID_ARG;
     int rc = drain_timing(h);
     if (rc != GFA2_OK) return rc;
-    if (count) *count = kGroups;
-    for (int g = 0; g < kGroups && g < cap; ++g) {
-        if (names) names[g] = kGroupNames[g];
-        if (ms) ms[g] = h->timed_iters ? h->acc_ms[g] / (double)h->timed_iters : 0.0;
+    // the groups the last timed iterations measured
+    int k = 0;
+    for (int g = 0; g < kGroups; ++g) {
+        if (h->timing == GFA2_TIMING_WALK && g != kWalkGroup) continue;
+        if (k < cap) {
+            if (names) names[k] = kGroupNames[g];
+            if (ms) ms[k] = h->timed_iters ? h->acc_ms[g] / (double)h->timed_iters : 0.0;
+        }
+        ++k;
     }
+    if (count) *count = k < cap ? k : cap;
     return GFA2_OK;
 }
 
@@ -1116,8 +1129,8 @@ int gfa2_autotune(gfa2_handle h, int32_t iters, gfa2_launch_config* best) {
     };
     hipError_t e = copy_all(true);
     hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (e == hipSuccess) e = hipEventCreate(&t0);
-    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t0, hipEventDisableSystemFence);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t1, hipEventDisableSystemFence);
     const LaunchConfig keep = h->tune;
     LaunchConfig win = keep;
     float win_ms = 1e30f;

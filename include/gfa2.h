@@ -155,9 +155,13 @@ int gfa2_get_tree(gfa2_handle h, int32_t cap, int32_t* ncells, int32_t* level, i
 int gfa2_get_stats(gfa2_handle h, gfa2_stats* s);
 
 /* Per-kernel-group timing with HIP events on the handle's stream (no counterpart in the
- * reference). When on, every iteration records events between launch groups. */
+ * reference). on = GFA2_TIMING_GROUPS: every iteration records events between all launch
+ * groups; GFA2_TIMING_WALK: only the pair around the force walk (the least added work);
+ * 0: off. The events carry no system-scope fence. */
+#define GFA2_TIMING_GROUPS 1
+#define GFA2_TIMING_WALK 2
 int gfa2_set_timing(gfa2_handle h, int32_t on);
-/* Average milliseconds per iteration of each group since timing was enabled:
+/* Average milliseconds per iteration of each measured group since timing was enabled:
  * names[i] (static strings), ms[i]; *count receives the number of groups (<= cap). */
 int gfa2_get_timing(gfa2_handle h, int32_t cap, int32_t* count, const char** names, double* ms);
 

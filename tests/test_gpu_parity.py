@@ -296,6 +296,32 @@ def test_streams_in_a_captured_graph_replay_bit_for_bit(mode):
     np.testing.assert_array_equal(out[0].view(np.uint32), out[2].view(np.uint32))
 
 
+def test_timing_modes_leave_the_trajectory_alone():
+    """Events around every group, around the walk only, or none: the same positions bit for
+    bit; the walk-only mode reports the walk alone; an unknown mode is refused."""
+    print("case timing modes", flush=True)
+    g, pos = make_case("rmat", 46)
+    out = []
+    for mode in (False, True, "walk"):
+        eng = Engine(g.n, g.src, g.dst, scale=10.0)
+        eng.set_launch_config(256, 4, 2, 1)
+        eng.set_positions(pos)
+        eng.set_timing(mode)
+        eng.step(8)
+        eng.sync()
+        t = eng.timing()
+        if mode == "walk":
+            assert list(t) == ["walk"] and t["walk"] > 0
+        elif mode:
+            assert len(t) == 8 and t["walk"] > 0 and t["sort"] > 0
+        out.append(eng.positions())
+        with pytest.raises(GFA2Error):
+            eng.set_timing(3)
+        eng.close()
+    for o in out[1:]:
+        np.testing.assert_array_equal(out[0].view(np.uint32), o.view(np.uint32))
+
+
 def test_autotune_restores_the_trajectory():
     print("case autotune", flush=True)
     g, pos = make_case("ba", 45)
