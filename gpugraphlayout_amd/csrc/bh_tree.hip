@@ -460,17 +460,21 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int* __restrict__ blocksum
     }
 }
 
-// S (exclusive prefix of opened cells), body records and brec.
+// S (exclusive prefix of opened cells), body records and brec. With `fused` (few tiles), the
+// tile offsets are not scanned by a launch of their own: every block sums the totals of the
+// tiles before it (k_lcp's, at most kFuseTiles of them, read from L2), its writes are bounded
+// by its own extent, and the blocks at the ends take over the scan launch's state updates.
 __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __restrict__ lcp,
                                                          const int* __restrict__ sg, int n,
                                                          const int* __restrict__ blockoff,
                                                          int* __restrict__ S,
                                                          const float4* __restrict__ sbody,
                                                          float4* __restrict__ rec, int* __restrict__ brec,
-                                                         const DevState* __restrict__ st,
+                                                         DevState* __restrict__ st,
                                                          const float4* __restrict__ seg_root,
                                                          const Moment* __restrict__ bmass, Moment* __restrict__ pmass,
-                                                         int* __restrict__ cell_pos, int max_cells) {
+                                                         int* __restrict__ cell_pos, int max_cells, int fused,
+                                                         int cap_records, int* __restrict__ rec_end) {
     // the tile's sorted bodies and graph ids (one before the tile), read coalesced into LDS
     __shared__ float4 sbd[kTreeTile];
     __shared__ int ssg[kTreeTile + 1];
@@ -495,8 +499,43 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
         sum += s[k];
     }
     int total;
-    int run = block_exclusive_scan(sum, &total) + blockoff[blockIdx.x];
-    const bool ok = st->err == 0;
+    int run = block_exclusive_scan(sum, &total);
+    int off;
+    Moment moff{0ull, 0ull, 0ull, 0ull, 0ull};
+    if (fused) {
+        int acc = 0;
+        Moment macc{0ull, 0ull, 0ull, 0ull, 0ull};
+        for (int t = threadIdx.x; t < (int)blockIdx.x; t += kTreeThreads) {
+            acc += blockoff[t];
+            if (bmass) macc = mo_add(macc, bmass[t]);
+        }
+        (void)block_exclusive_scan(acc, &off);
+        if (bmass) (void)block_exclusive_scan_mo<kTreeThreads>(macc, &moff);
+    } else {
+        off = blockoff[blockIdx.x];
+        if (bmass) moff = bmass[blockIdx.x];
+    }
+    run += off;
+    // records written here: below min(base + tile, n) + off + total (the cells before and in
+    // this tile), within capacity whenever the whole tree is
+    const bool ok = st->err == 0 && (!fused || min(base + kTreeTile, n) + off + total <= cap_records);
+    if (fused && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) {
+        const int all = off + total;
+        st->ncells = all;
+        st->nrecords = n + all;
+        *rec_end = n + all;
+        if (n + all > cap_records) st->err |= kErrTreeOverflow;
+    }
+    if (fused && blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            st->max_level = 0;
+            st->nbig = 0;
+        }
+        if (threadIdx.x <= kMaxLevels) {
+            st->level_count[threadIdx.x] = 0;
+            st->level_cursor[threadIdx.x] = 0;
+        }
+    }
     if (pmass) {
         // exclusive prefix of body mass sums over sorted positions
         Moment mine{0ull, 0ull, 0ull, 0ull, 0ull};
@@ -504,7 +543,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
         for (int k = 0; k < kTreeItems; ++k)
             if (i0 + k < n) mine = mo_add(mine, body_moment(bd[k], sg ? ssg[i0 + k - base + 1] : 0, seg_root));
         Moment tot;
-        Moment pre = mo_add(block_exclusive_scan_mo<kTreeThreads>(mine, &tot), bmass[blockIdx.x]);
+        Moment pre = mo_add(block_exclusive_scan_mo<kTreeThreads>(mine, &tot), moff);
 #pragma unroll
         for (int k = 0; k < kTreeItems; ++k) {
             const int i = i0 + k;
@@ -831,18 +870,21 @@ void launch_tree(Bufs& b, hipStream_t s) {
     Moment* pm = b.int_mass ? b.pmass : nullptr;
     k_lcp<<<tiles, kTreeThreads, 0, s>>>(b.sort.keys[0], sg, n, b.lcp, b.blocksum, b.sort.vals[0], b.pm,
                                          b.seg_root, bm, b.sbody);
+    // up to kFuseTiles tiles, k_prefix sums the tile totals itself
+    const bool fused = tiles <= kFuseTiles;
     if (tiles > 8 * kTileChunk) {
         const int chunks = (tiles + kTileChunk - 1) / kTileChunk;
         Moment* cm = b.int_mass ? b.chunk_mass : nullptr;
         k_scan_up<<<chunks, kTileChunk, 0, s>>>(b.blocksum, bm, tiles, b.chunk_sum, cm);
         k_scan_blocks<<<1, 1024, 0, s>>>(b.chunk_sum, chunks, n, b.cap_records, b.rec_begin + b.nseg, b.st, cm);
         k_scan_down<<<chunks, kTileChunk, 0, s>>>(b.blocksum, bm, tiles, b.chunk_sum, cm);
-    } else {
+    } else if (!fused) {
         k_scan_blocks<<<1, 1024, 0, s>>>(b.blocksum, tiles, n, b.cap_records, b.rec_begin + b.nseg, b.st, bm);
     }
     const int max_cells = b.cap_records - n;
     k_prefix<<<tiles, kTreeThreads, 0, s>>>(b.lcp, sg, n, b.blocksum, b.S, b.sbody, b.rec, b.brec, b.st, b.seg_root,
-                                            bm, pm, b.cell_rec, max_cells);
+                                            bm, pm, b.cell_rec, max_cells, fused ? 1 : 0, b.cap_records,
+                                            b.rec_begin + b.nseg);
     // grid-stride over the cells (their number is known on the device only; at most
     // max_cells, typically below n)
     const int node_blocks = (n + kTreeThreads - 1) / kTreeThreads;

@@ -209,6 +209,8 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kTreeThreads = 256;
 constexpr int kTreeItems = 2;
 constexpr int kTreeTile = kTreeThreads * kTreeItems;
+// up to this many tiles k_prefix sums the tile totals before it itself (no scan launch)
+constexpr int kFuseTiles = 2048;
 constexpr int kTileChunk = 256;       // tree tiles per block of the two-level tile scan
 
 // Radix sort of (key, value) pairs on 64-bit keys, ascending and stable. Sorts

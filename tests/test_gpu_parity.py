@@ -441,6 +441,26 @@ def sorted_passes(T):
     return 8
 
 
+def test_tree_overflow_is_reported():
+    """More cells than the record capacity (collapsing pairs hang long cell chains; cell_factor
+    1 leaves room for about one cell per body): the iteration's kernels skip, sync reports the
+    overflow, and the positions are left as they were."""
+    print("case tree overflow", flush=True)
+    rng = np.random.default_rng(13)
+    npairs = 600
+    centres = rng.uniform(-300, 300, size=(npairs, 2))
+    offs = rng.normal(size=(npairs, 2)) * 1e-3
+    pos = np.concatenate([centres + offs, centres - offs]).astype(np.float32)
+    eng = Engine(2 * npairs, np.arange(npairs), np.arange(npairs) + npairs, cell_factor=1)
+    eng.set_positions(pos)
+    eng.step(1)
+    with pytest.raises(GFA2Error) as exc:
+        eng.sync()
+    assert exc.value.code == -5
+    np.testing.assert_array_equal(eng.positions(), pos)
+    eng.close()
+
+
 def test_sort_fixup_when_the_tree_deepens(oracle):
     """The path-key sort runs only the top digit passes the previous tree's cells call for; when
     the tree deepens in one iteration (pairs of bodies joined by an edge pull together), the
