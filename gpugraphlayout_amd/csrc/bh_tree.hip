@@ -100,14 +100,19 @@ __global__ __launch_bounds__(256) void k_bbox_partials(const float2* __restrict_
 }
 
 // Root box + path keys. vals = identity; pm = {x, y, mass} per node, so that k_lcp gathers one
-// 16-byte record per sorted body instead of a position and a mass.
+// 16-byte record per sorted body instead of a position and a mass. With dhist (a sort of one
+// launch per pass), also the digit counts of all 8 passes, added to the table the previous
+// iteration's fix-up cleared (the sort's own histogram launches are then skipped).
 __global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, const float* __restrict__ mass, int n,
                                               const float4* __restrict__ part, DevState* st,
                                               float4* __restrict__ seg_root,
                                               unsigned long long* __restrict__ k0, unsigned int* __restrict__ v0,
                                               unsigned long long* __restrict__ k1, unsigned int* __restrict__ v1,
-                                              float4* __restrict__ pm, int first) {
+                                              float4* __restrict__ pm, int first, unsigned int* __restrict__ dhist) {
     __shared__ float root[3];
+    __shared__ unsigned int h[8][256];
+    if (dhist)
+        for (int p = 0; p < 8; ++p) h[p][threadIdx.x] = 0u;
     // the buffer from which the sort passes that will run (8 less the skipped or unlaunched low
     // ones) end in buffer 0
     const bool odd = ((8 - max(st->sort_skip, first)) & 1) != 0;
@@ -140,6 +145,16 @@ __global__ __launch_bounds__(256) void k_keys(const float2* __restrict__ pos, co
         keys[i] = key;
         vals[i] = (unsigned)i;
         pm[i] = make_float4(p.x, p.y, mass[i], 0.0f);   // one gather per body for k_lcp
+        if (dhist)
+#pragma unroll
+            for (int d = 0; d < 8; ++d) atomicAdd(&h[d][(unsigned)(key >> (8 * d)) & 255u], 1u);
+    }
+    if (dhist) {
+        __syncthreads();
+        for (int d = 0; d < 8; ++d) {
+            const unsigned int c = h[d][threadIdx.x];
+            if (c) atomicAdd(&dhist[d * 256 + threadIdx.x], c);
+        }
     }
 }
 
@@ -859,7 +874,8 @@ void launch_keys(const Bufs& b, int first, hipStream_t s) {
     int blocks = (b.n + 255) / 256;
     if (blocks > b.grid_cap) blocks = b.grid_cap;
     k_keys<<<blocks, 256, 0, s>>>(b.pos, b.mass, b.n, b.bbox_part, b.st, b.seg_root, b.sort.keys[0], b.sort.vals[0],
-                                  b.sort.keys[1], b.sort.vals[1], b.pm, first);
+                                  b.sort.keys[1], b.sort.vals[1], b.pm, first,
+                                  b.sort.hist_in_keys ? b.sort.digit_hist : nullptr);
 }
 
 void launch_tree(Bufs& b, hipStream_t s) {
@@ -894,7 +910,7 @@ void launch_tree(Bufs& b, hipStream_t s) {
     // every record's info word is in place (cells: k_nodes, bodies: k_prefix)
     k_kids<<<b.kids_blocks, 256, 0, s>>>(b.rec, b.rcount, b.S, n, b.cell_rec, b.kids, b.kids_part, b.st);
     if (b.int_mass) {
-        k_big<<<1, 256, 0, s>>>(b.rec, b.big_rc, b.big_lvl, b.st);
+        if (!b.no_big) k_big<<<1, 256, 0, s>>>(b.rec, b.big_rc, b.big_lvl, b.st);
         return;
     }
     k_levels<<<(max_cells + 255) / 256, 256, 0, s>>>(b.cell_rec, b.cell_lvl, b.level_list, b.st);

@@ -211,8 +211,8 @@ void enqueue_iteration(gfa2_engine* h, hipEvent_t* ev) {
             // are made up for by the fix-up, so the order is the full sort's whatever the depth)
             // a single graph's keys start where the passes that run end in buffer 0
             const bool lead = b.nseg == 1;
-            radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st, h->sort_first, lead);
-            launch_sort_fixup(b.sort, b.n, b.st, s, h->sort_first, lead);
+            radix_sort_pairs(b.sort, b.n, 0, 64, s, b.st, h->sort_first, lead, b.sort.hist_in_keys);
+            launch_sort_fixup(b.sort, b.n, b.st, s, h->sort_first, lead, nullptr, b.sort.hist_in_keys);
             if (b.nseg > 1) launch_seg_graph_sort(b, s);
         }
     }
@@ -547,6 +547,10 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
     GFA2_ALLOC(b.sort.hist, (size_t)256 * sort_tiles);
     GFA2_ALLOC(b.sort.rowtot, 256);
     GFA2_ALLOC(b.sort.digit_hist, 8 * 256);
+    // one graph sorted with one launch per pass: k_keys counts the digits into a table that
+    // starts clear and that every iteration's fix-up clears again
+    b.sort.hist_in_keys = nseg == 1 && sort_onesweep(b.sort, n);
+    GFA2_TRY(hipMemsetAsync(b.sort.digit_hist, 0, sizeof(unsigned int) * 8 * 256, h->stream));
     GFA2_ALLOC(b.sort.status, (size_t)256 * sort_tiles);
     GFA2_ALLOC(b.sort.ticket, 1);
     // look-back state starts empty: no published tile, ticket 0
@@ -614,6 +618,8 @@ int set_graph_impl(gfa2_engine* h, int32_t n, const int64_t* rowptr, const int32
             total += m[v];
         }
         b.int_mass = whole && total < 16777216.0 * 62.0;
+        // no cell (a part of one graph) can reach 2^24
+        b.no_big = total < 16777216.0;
     }
     if (b.int_mass) {
         GFA2_ALLOC(b.pmass, (size_t)n + 1);

@@ -81,6 +81,7 @@ struct SortBuffers {
     unsigned int* rowtot;    // 256
     bool onesweep = true;    // one launch per pass with decoupled look-back
     unsigned int* digit_hist = nullptr;   // 8 x 256: digit totals of every pass
+    bool hist_in_keys = false;            // k_keys counts the digits; the fix-up clears the table
     unsigned int* status = nullptr;       // tiles x 256: published tile counts / prefixes
     unsigned long long* ticket = nullptr; // tile tickets, never reset while the tile count holds
     int* err = nullptr;                   // the instance's error word: kErrSortWait is raised there if a
@@ -181,6 +182,7 @@ struct Bufs {
     // the same float mass in every summation order, so it is summarised from prefix sums;
     // only heavier cells are summed in quadrant order.
     bool int_mass = false;
+    bool no_big = false;              // total mass below 2^24: no cell is summed in quadrant order
     Moment* pmass = nullptr;          // n + 1
     Moment* bmass = nullptr;          // per tree tile
     int* chunk_sum = nullptr;         // per 256 tree tiles (the two-level tile scan)
@@ -222,14 +224,18 @@ constexpr int kTileChunk = 256;       // tree tiles per block of the two-level t
 // lead: the keys were written to the buffer from which the passes that run end in buffer 0
 // (key_buffer); otherwise they start in buffer 0.
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s,
-                      const DevState* skip = nullptr, int first = 0, bool lead = false);
+                      const DevState* skip = nullptr, int first = 0, bool lead = false, bool hist_ready = false);
+// Whether radix_sort_pairs sorts n pairs with one launch per pass (the digit histograms of
+// every pass taken first).
+bool sort_onesweep(const SortBuffers& buf, int n);
 // After radix_sort_pairs with a device skip: orders the runs of pairs that agree in every
 // sorted bit by (whole key, value) and leaves the whole result in keys[0]/vals[0] (the passes
 // that ran may have ended in the other buffer).
 // seg: the graph of each sorted position when every graph was sorted in its own range (runs
 // then stop at graph boundaries); null for one graph or a global sort.
+// clear_hist: also clears buf.digit_hist for the next iteration's k_keys.
 void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first = 0,
-                       bool lead = false, const int* seg = nullptr);
+                       bool lead = false, const int* seg = nullptr, bool clear_hist = false);
 // Passes of the path-key sort below which the fix-up finishes the order: the smallest p whose
 // 4p-level cells of the previous tree hold at most this many bodies.
 constexpr int kSortRunMax = 16;

@@ -498,7 +498,11 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
                                                     const unsigned long long* __restrict__ k1,
                                                     const unsigned int* __restrict__ v1, int n,
                                                     const DevState* __restrict__ st, int first, int lead,
-                                                    const int* __restrict__ seg) {
+                                                    const int* __restrict__ seg,
+                                                    unsigned int* __restrict__ clear_hist) {
+    // the passes are done with the digit table: cleared for the next iteration's k_keys
+    if (clear_hist && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) clear_hist[i] = 0u;
     const int skip = passes_below(st, first);
     if (skip == 0) return;                       // all 8 passes ran: sorted, ties by value, in buffer 0
     // the passes that ran ended in buffer 1 (unless the keys started where they end in buffer 0)
@@ -541,12 +545,16 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
 }  // namespace
 
 void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first, bool lead,
-                       const int* seg) {
+                       const int* seg, bool clear_hist) {
     if (n < 1) return;
     int blocks = (n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
     k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], n, st, first,
-                                        lead ? 8 : 0, seg);
+                                        lead ? 8 : 0, seg, clear_hist ? buf.digit_hist : nullptr);
+}
+
+bool sort_onesweep(const SortBuffers& buf, int n) {
+    return buf.onesweep && (n + kSortTile - 1) / kSortTile <= kOnesweepMaxTiles;
 }
 
 void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* order, int nseg, hipStream_t s,
@@ -557,7 +565,7 @@ void segmented_sort_pairs(SortBuffers& buf, const int* node_off, const int* orde
 }
 
 void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipStream_t s, const DevState* skip,
-                      int first, bool lead_in) {
+                      int first, bool lead_in, bool hist_ready) {
     if (n <= 1) return;
     const int tiles = (n + kSortTile - 1) / kSortTile;
     const int npasses = (end_bit - begin_bit + 7) / 8;
@@ -569,10 +577,12 @@ void radix_sort_pairs(SortBuffers& buf, int n, int begin_bit, int end_bit, hipSt
     // One launch per pass pays off while the look-back chains stay short; with hundreds of
     // tiles resident at once the chains grow with the tile count and three launches per pass win.
     if (buf.onesweep && tiles <= kOnesweepMaxTiles) {
-        k_sort_zero<<<1, 256, 0, s>>>(buf.digit_hist, npasses);
-        int hb = (n + 255) / 256;
-        if (hb > 256) hb = 256;
-        k_sort_hist<<<hb, 256, 0, s>>>(k0, k1, n, begin_bit, npasses, buf.digit_hist, skip, first, lead);
+        if (!hist_ready) {
+            k_sort_zero<<<1, 256, 0, s>>>(buf.digit_hist, npasses);
+            int hb = (n + 255) / 256;
+            if (hb > 256) hb = 256;
+            k_sort_hist<<<hb, 256, 0, s>>>(k0, k1, n, begin_bit, npasses, buf.digit_hist, skip, first, lead);
+        }
         for (int p = first; p < npasses; ++p)
             k_onesweep<<<tiles, kSortThreads, 0, s>>>(k0, v0, k1, v1, n, begin_bit + 8 * p, buf.digit_hist + 256 * p,
                                                       buf.status, buf.ticket, tiles, buf.err, skip, first, lead, p);
