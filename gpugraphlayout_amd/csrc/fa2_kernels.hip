@@ -299,7 +299,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 template <int G, int BT>
 __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, const int4* __restrict__ kids,
-                                              const int* __restrict__ brec,
+                                              const float4* __restrict__ sbody,
                                               const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
                                               int ngroups, const int* __restrict__ node_off,
                                               const int* __restrict__ rec_begin, const float4* __restrict__ seg_root,
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
     const int bl = (G == 32) ? (lane & 31) : lane;    // this lane's body within the group
     const bool bact = bl < nact;
     const int k = grp.x + min(bl, nact - 1);          // inactive lanes duplicate the last body
-    const float4 me = rec[brec[k]];
+    const float4 me = sbody[k];     // = the body's record, read without the record-index hop
     const float px = me.x, py = me.y, mi = me.z;
     const unsigned int id = perm[k];
     // the group's bounding box (inactive lanes duplicate an active body) and its first body
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(BT) void k_walk(const float4* __restrict__ rec, con
 // the reduction order of the fused epilogue, so both give the same bits. Groups of G = 32 take
 // half a wave each (the xor offsets of the reduction stay inside a half).
 template <int G>
-__global__ __launch_bounds__(256) void k_combine(const float4* __restrict__ rec, const int* __restrict__ brec,
+__global__ __launch_bounds__(256) void k_combine(const float4* __restrict__ sbody,
                                                  const unsigned int* __restrict__ perm, const int2* __restrict__ groups,
                                                  int ngroups, const int* __restrict__ node_off,
                                                  const float2* __restrict__ fga, const float2* __restrict__ fprev,
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(256) void k_combine(const float4* __restrict__ rec,
         if (own) {
             const int k = grp.x + bl;
             id = perm[k];
-            mi = rec[brec[k]].z;
+            mi = sbody[k].z;
         }
     }
     float sw = 0.0f, tr = 0.0f;
@@ -876,7 +876,7 @@ template <int G, int BT>
 void walk_launch(const Bufs& b, const Params& p, bool split, hipStream_t s) {
     constexpr int W = BT / 64;     // one group per wave
     const int blocks = (b.ngroups + W - 1) / W;
-    k_walk<G, BT><<<blocks < 1 ? 1 : blocks, BT, 0, s>>>(b.rec, b.kids, b.brec, b.sort.vals[0], b.groups, b.ngroups,
+    k_walk<G, BT><<<blocks < 1 ? 1 : blocks, BT, 0, s>>>(b.rec, b.kids, b.sbody, b.sort.vals[0], b.groups, b.ngroups,
                                                           b.node_off, b.rec_begin, b.seg_root, split ? nullptr : b.fga,
                                                           b.fprev, b.f, split ? nullptr : b.rep, b.grp_part, p.k_r,
                                                           p.itolsq, p.epssq, b.st);
@@ -929,10 +929,10 @@ void launch_combine(const Bufs& b, const Params& p, hipStream_t s) {
     const int per_block = 4 * (64 / p.group);
     const int blocks = (b.ngroups + per_block - 1) / per_block;
     if (p.group == 64)
-        k_combine<64><<<blocks < 1 ? 1 : blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups,
+        k_combine<64><<<blocks < 1 ? 1 : blocks, 256, 0, s>>>(b.sbody, b.sort.vals[0], b.groups, b.ngroups,
                                                               b.node_off, b.fga, b.fprev, b.f, b.rep, b.grp_part, b.st);
     else
-        k_combine<32><<<blocks < 1 ? 1 : blocks, 256, 0, s>>>(b.rec, b.brec, b.sort.vals[0], b.groups, b.ngroups,
+        k_combine<32><<<blocks < 1 ? 1 : blocks, 256, 0, s>>>(b.sbody, b.sort.vals[0], b.groups, b.ngroups,
                                                               b.node_off, b.fga, b.fprev, b.f, b.rep, b.grp_part, b.st);
 }
 
