@@ -590,9 +590,10 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
 // k_prefix) and the cell's record index on exit (the list k_kids walks; on the fractional-mass
 // path, with cell_lvl, also the per-level summaries' list). The cells opened at
 // position i have levels lprev+1 .. lcp_i in ord order; each finds its extent end by a
-// galloping search over the keys (sorted within a graph, so a cell's bodies are contiguous).
-// One thread per cell keeps the long searches of the top levels, which all start at a
-// graph's first position, in parallel.
+// galloping search over the keys (sorted within a graph, so a cell's bodies are contiguous),
+// eight probes per step with their loads in flight together, so that the top levels' long
+// searches take log8 rather than log2 dependent steps (a graph's root needs none: it ends
+// where the graph does). One thread per cell keeps those searches in parallel.
 __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long* __restrict__ keys,
                                                         const signed char* __restrict__ lcp,
                                                         const int* __restrict__ sg, int n,
@@ -601,7 +602,8 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
                                                         unsigned char* __restrict__ cell_lvl, int* __restrict__ rec_begin,
                                                         DevState* st, const Moment* __restrict__ pmass,
                                                         const float4* __restrict__ seg_root, int* __restrict__ big_rc,
-                                                        unsigned char* __restrict__ big_lvl) {
+                                                        unsigned char* __restrict__ big_lvl,
+                                                        const int* __restrict__ node_off) {
     __shared__ int hist[kMaxLevels + 1];
     __shared__ int lmax;
     if (threadIdx.x <= kMaxLevels) hist[threadIdx.x] = 0;
@@ -615,20 +617,52 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
         const int L = lcp_prev(lcp, sg, i) + 1 + (ord - Si);
         const unsigned long long ki = keys[i];
         const int gi = sg ? sg[i] : 0;
-        // extent end: largest j of the same graph with lcp(k_i, k_j) >= L
-        int lo = i;
-        int step = 1;
-        int hi = lo + 1;
-        while (hi < n && (!sg || sg[hi] == gi) && lcp_levels(ki, keys[hi]) >= L) {
-            lo = hi;
-            step <<= 1;
-            hi = lo + step;
-        }
-        if (hi > n) hi = n;
-        while (hi - lo > 1) {
-            int mid = lo + ((hi - lo) >> 1);
-            if ((!sg || sg[mid] == gi) && lcp_levels(ki, keys[mid]) >= L) lo = mid;
-            else hi = mid;
+        // extent end: largest j < lim (the graph's end) with lcp(k_i, k_j) >= L; the test is
+        // true up to the end and false after it
+        const int lim = sg ? node_off[gi + 1] : n;
+        int lo = i;                   // inside
+        if (L == 0) {
+            lo = lim - 1;             // a graph's root holds the whole graph
+        } else {
+            long long step = 1;
+            int hi = lim;             // outside (or the graph's end)
+            while (true) {
+                // probes lo + step, ..., lo + 8 step; the leading ones inside move lo
+                unsigned long long kk[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const long long pq = lo + step * (q + 1);
+                    kk[q] = pq < lim ? keys[pq] : 0ull;
+                }
+                int in = 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (in == q && lo + step * (q + 1) < lim && lcp_levels(ki, kk[q]) >= L) in = q + 1;
+                if (in < 8) {
+                    hi = (int)min((long long)lim, lo + step * (in + 1));
+                    lo = (int)(lo + step * in);
+                    break;
+                }
+                lo = (int)(lo + step * 8);
+                step *= 8;
+            }
+            // eight-way search between lo (inside) and hi (outside)
+            while (hi - lo > 1) {
+                const int st8 = (hi - lo + 7) >> 3;
+                unsigned long long kk[7];
+#pragma unroll
+                for (int q = 0; q < 7; ++q) {
+                    const int pq = lo + st8 * (q + 1);
+                    kk[q] = pq < hi ? keys[pq] : 0ull;
+                }
+                int in = 0;
+#pragma unroll
+                for (int q = 0; q < 7; ++q)
+                    if (in == q && lo + st8 * (q + 1) < hi && lcp_levels(ki, kk[q]) >= L) in = q + 1;
+                const int nlo = lo + st8 * in;
+                hi = min(hi, lo + st8 * (in + 1));
+                lo = nlo;
+            }
         }
         const int b = lo;
         const int rc = i + ord;
@@ -906,7 +940,7 @@ void launch_tree(Bufs& b, hipStream_t s) {
     const int node_blocks = (n + kTreeThreads - 1) / kTreeThreads;
     k_nodes<<<node_blocks, kTreeThreads, 0, s>>>(b.sort.keys[0], b.lcp, sg, n, b.S, b.rec, b.rcount, b.cell_rec,
                                                  b.cell_lvl, b.rec_begin, b.st, pm, b.seg_root, b.big_rc,
-                                                 b.big_lvl);
+                                                 b.big_lvl, b.node_off);
     // every record's info word is in place (cells: k_nodes, bodies: k_prefix)
     k_kids<<<b.kids_blocks, 256, 0, s>>>(b.rec, b.rcount, b.S, n, b.cell_rec, b.kids, b.kids_part, b.st);
     if (b.int_mass) {

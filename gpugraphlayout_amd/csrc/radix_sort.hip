@@ -547,8 +547,10 @@ __global__ __launch_bounds__(256) void k_sort_fixup(unsigned long long* __restri
 void launch_sort_fixup(SortBuffers& buf, int n, const DevState* st, hipStream_t s, int first, bool lead,
                        const int* seg, bool clear_hist) {
     if (n < 1) return;
+    // one position per thread: a thread that finds a run's first position sorts the run with
+    // dependent loads, so positions taken one after another by a thread add up their latencies
     int blocks = (n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
+    if (blocks > 65535) blocks = 65535;
     k_sort_fixup<<<blocks, 256, 0, s>>>(buf.keys[0], buf.vals[0], buf.keys[1], buf.vals[1], n, st, first,
                                         lead ? 8 : 0, seg, clear_hist ? buf.digit_hist : nullptr);
 }
