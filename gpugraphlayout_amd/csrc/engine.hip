@@ -297,9 +297,17 @@ void set_sort_first(gfa2_engine* h, int first) {
     invalidate_graph(h);
 }
 
+// New positions: the next sort is a full one, and so are those of the following
+// kFullSortsAfterNew iterations. The first steps from a fresh layout move bodies the most, so a
+// tree would underestimate the next one's tied runs (at R-MAT 2^24 from a random start, the
+// second iteration's runs took the fix-up 31 ms).
+constexpr int kFullSortsAfterNew = 3;
+
 hipError_t full_sort_next(gfa2_engine* h) {
     set_sort_first(h, 0);
-    return hipMemsetAsync(&h->b.st->sort_skip, 0, sizeof(int), h->stream);
+    hipError_t e = hipMemsetAsync(&h->b.st->sort_skip, 0, sizeof(int), h->stream);
+    if (e == hipSuccess) e = hipMemsetD32Async((hipDeviceptr_t)&h->b.st->full_sorts, kFullSortsAfterNew, 1, h->stream);
+    return e;
 }
 
 // The ABI status of the device error word (tree overflow first: it is the one a caller can

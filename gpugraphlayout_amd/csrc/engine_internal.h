@@ -50,6 +50,7 @@ struct DevState {
     int nbbox;          // number of valid bounding-box partials
     int nbig;           // heavy cells listed for the in-order summary
     int sort_skip;      // low 8-bit digit passes the next path-key sort skips (0: a full sort)
+    int full_sorts;     // iterations after new positions whose next sort is still a full one
     int runmax[8];      // largest body count of a cell at level 4, 8, ..., 32 (the last tree)
     int level_count[kMaxLevels + 1];
     int level_offset[kMaxLevels + 2];

@@ -699,6 +699,10 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const WalkPart* __restr
             // kSortRunMax bodies each
             int p = 1;
             while (p < 8 && rm[p - 1] > kSortRunMax) ++p;
+            if (st->full_sorts > 0) {
+                st->full_sorts -= 1;      // still settling after new positions
+                p = 8;
+            }
             st->sort_skip = 8 - p;
             for (int q = 0; q < 8; ++q) st->runmax[q] = rm[q];
         }

@@ -430,6 +430,10 @@ def expected_order(p):
     return np.lexsort((np.arange(len(p)), key)), key
 
 
+# New positions: the engine's next sort and those of the following 3 iterations are full ones.
+FULL_SORTS_AFTER_NEW = 3
+
+
 def sorted_passes(T):
     """The digit passes the next path-key sort runs, restated from a tree: the fewest p whose
     level-4p cells hold at most 16 bodies each (the pairs still tied after p passes are the
@@ -466,7 +470,7 @@ def test_sort_fixup_when_the_tree_deepens(oracle):
     the tree deepens in one iteration (pairs of bodies joined by an edge pull together), the
     fix-up must still give the full sort's order. Every iteration's body order equals a numpy
     sort of the restated keys, its tree is the oracle's where no bodies coincide, and at least
-    one iteration needed the fix-up."""
+    one iteration needed the fix-up (after the full sorts that follow new positions)."""
     print("case sort fix-up on a deepening tree", flush=True)
     rng = np.random.default_rng(12)
     npairs = 600
@@ -480,7 +484,7 @@ def test_sort_fixup_when_the_tree_deepens(oracle):
     eng = Engine(2 * npairs, src, dst, scale=1e-6, gravity=0.0, strong_gravity=1, cell_factor=40)
     eng.set_positions(pos)
     prev_tree, exercised, checked = None, 0, 0
-    for it in range(6):
+    for it in range(10):
         p = eng.positions()
         eng.step(1)
         eng.sync()
@@ -489,8 +493,9 @@ def test_sort_fixup_when_the_tree_deepens(oracle):
         order, key = expected_order(p)
         np.testing.assert_array_equal(T["perm"], order, err_msg=f"iteration {it}")
         if prev_tree is not None:
-            # the bits this iteration's sort ordered by: chosen from the previous tree
-            bits = 8 * sorted_passes(prev_tree)
+            # the bits this iteration's sort ordered by: chosen from the previous tree once
+            # the full sorts after new positions are over
+            bits = 64 if it <= FULL_SORTS_AFTER_NEW else 8 * sorted_passes(prev_tree)
             k = key[order]
             shift = np.uint64(64 - bits)
             exercised += int(bits < 64 and np.any((k[1:] >> shift) == (k[:-1] >> shift)))
