@@ -591,9 +591,8 @@ __global__ __launch_bounds__(kTreeThreads) void k_prefix(const signed char* __re
 // path, with cell_lvl, also the per-level summaries' list). The cells opened at
 // position i have levels lprev+1 .. lcp_i in ord order; each finds its extent end by a
 // galloping search over the keys (sorted within a graph, so a cell's bodies are contiguous),
-// eight probes per step with their loads in flight together, so that the top levels' long
-// searches take log8 rather than log2 dependent steps (a graph's root needs none: it ends
-// where the graph does). One thread per cell keeps those searches in parallel.
+// bounded by the graph's end (a graph's root needs no search: it ends where the graph does).
+// One thread per cell keeps the long searches of the top levels in parallel.
 __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long* __restrict__ keys,
                                                         const signed char* __restrict__ lcp,
                                                         const int* __restrict__ sg, int n,
@@ -624,44 +623,18 @@ __global__ __launch_bounds__(kTreeThreads) void k_nodes(const unsigned long long
         if (L == 0) {
             lo = lim - 1;             // a graph's root holds the whole graph
         } else {
-            long long step = 1;
-            int hi = lim;             // outside (or the graph's end)
-            while (true) {
-                // probes lo + step, ..., lo + 8 step; the leading ones inside move lo
-                unsigned long long kk[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const long long pq = lo + step * (q + 1);
-                    kk[q] = pq < lim ? keys[pq] : 0ull;
-                }
-                int in = 0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (in == q && lo + step * (q + 1) < lim && lcp_levels(ki, kk[q]) >= L) in = q + 1;
-                if (in < 8) {
-                    hi = (int)min((long long)lim, lo + step * (in + 1));
-                    lo = (int)(lo + step * in);
-                    break;
-                }
-                lo = (int)(lo + step * 8);
-                step *= 8;
+            int step = 1;
+            int hi = lo + 1;
+            while (hi < lim && lcp_levels(ki, keys[hi]) >= L) {
+                lo = hi;
+                step <<= 1;
+                hi = lo + step;
             }
-            // eight-way search between lo (inside) and hi (outside)
+            if (hi > lim) hi = lim;
             while (hi - lo > 1) {
-                const int st8 = (hi - lo + 7) >> 3;
-                unsigned long long kk[7];
-#pragma unroll
-                for (int q = 0; q < 7; ++q) {
-                    const int pq = lo + st8 * (q + 1);
-                    kk[q] = pq < hi ? keys[pq] : 0ull;
-                }
-                int in = 0;
-#pragma unroll
-                for (int q = 0; q < 7; ++q)
-                    if (in == q && lo + st8 * (q + 1) < hi && lcp_levels(ki, kk[q]) >= L) in = q + 1;
-                const int nlo = lo + st8 * in;
-                hi = min(hi, lo + st8 * (in + 1));
-                lo = nlo;
+                const int mid = lo + ((hi - lo) >> 1);
+                if (lcp_levels(ki, keys[mid]) >= L) lo = mid;
+                else hi = mid;
             }
         }
         const int b = lo;
