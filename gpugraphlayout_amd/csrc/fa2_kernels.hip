@@ -55,17 +55,24 @@ __device__ __forceinline__ void attract_term(float2 p, float2 q, int linlog, con
 // a time: their index and position loads are all issued before the first term is added (the
 // loads past e1 read node 0 and are not added), so a row's tail costs no more round trips to
 // memory than a full group of four.
-template <bool GEN, int S>
+// With HUB, the positions of the kHubCache highest-degree nodes come from the block's LDS copy
+// (hub), which spares the L2 the most requested lines.
+constexpr int kHubCache = 2048;
+
+template <bool GEN, int S, bool HUB>
 __device__ __forceinline__ void gather_terms(float2 p, const float2* __restrict__ pos_rank,
-                                             const int* __restrict__ col, int e, int e1, int linlog,
-                                             const float* __restrict__ ew, float& ax, float& ay) {
+                                             const float2* hub, const int* __restrict__ col, int e, int e1,
+                                             int linlog, const float* __restrict__ ew, float& ax, float& ay) {
     for (; e < e1; e += 4 * S) {
         int j[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) j[u] = e + u * S < e1 ? col[e + u * S] : 0;
         float2 q[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = pos_rank[j[u]];
+        for (int u = 0; u < 4; ++u) {
+            if (HUB && j[u] < kHubCache) q[u] = hub[j[u]];
+            else q[u] = pos_rank[j[u]];
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (e + u * S < e1) attract_term<GEN>(p, q[u], linlog, ew, e + u * S, ax, ay);
@@ -95,9 +102,9 @@ __global__ __launch_bounds__(256) void k_rank_pos(const float2* __restrict__ pos
 // of virtual blocks (light, medium, chunks) that the grid strides over: one launch block per
 // virtual block by default, or a smaller grid that leaves room on every compute unit for
 // kernels running beside it on another stream. Every row's sum is the same whatever the grid.
-template <bool GEN>
+template <bool GEN, bool HUB>
 __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos, const float* __restrict__ mass,
-                                                 const float2* __restrict__ pos_rank,
+                                                 int n, const float2* __restrict__ pos_rank,
                                                  const int* __restrict__ rowptr, const int* __restrict__ col,
                                                  const int* __restrict__ lists, int nlight, int nmedium,
                                                  const int2* __restrict__ chunks, int nchunks,
@@ -106,7 +113,12 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
                                                  float2* __restrict__ part, int linlog,
                                                  const float* __restrict__ ew) {
     __shared__ float2 red[256];
+    __shared__ float2 hub[HUB ? kHubCache : 1];
     const int tid = threadIdx.x;
+    if constexpr (HUB) {
+        for (int i = tid; i < kHubCache && i < n; i += 256) hub[i] = pos_rank[i];
+        __syncthreads();
+    }
     const int total = light_blocks + medium_blocks + nchunks;
     for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {   // uniform per block
         int blk = vb;
@@ -117,7 +129,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
                 const float2 p = pos[v];
                 float ax = 0.0f, ay = 0.0f;
                 const int e1 = rowptr[v + 1];
-                gather_terms<GEN, 1>(p, pos_rank, col, rowptr[v], e1, linlog, ew, ax, ay);
+                gather_terms<GEN, 1, HUB>(p, pos_rank, hub, col, rowptr[v], e1, linlog, ew, ax, ay);
                 const float2 g = gravity(p, mass[v], strong, k_g);
                 fga[v] = make_float2(g.x + ax, g.y + ay);
             }
@@ -132,7 +144,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
                 const float2 p = pos[v];
                 float ax = 0.0f, ay = 0.0f;
                 const int e1 = rowptr[v + 1];
-                gather_terms<GEN, 64>(p, pos_rank, col, rowptr[v] + lane, e1, linlog, ew, ax, ay);
+                gather_terms<GEN, 64, HUB>(p, pos_rank, hub, col, rowptr[v] + lane, e1, linlog, ew, ax, ay);
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) {
                     ax += __shfl_xor(ax, o, 64);
@@ -151,7 +163,7 @@ __global__ __launch_bounds__(256) void k_attract(const float2* __restrict__ pos,
         const float2 p = pos[v];
         float ax = 0.0f, ay = 0.0f;
         const int e1 = min(ch.y + kAttrChunk, rowptr[v + 1]);
-        gather_terms<GEN, 256>(p, pos_rank, col, ch.y + tid, e1, linlog, ew, ax, ay);
+        gather_terms<GEN, 256, HUB>(p, pos_rank, hub, col, ch.y + tid, e1, linlog, ew, ax, ay);
         red[tid] = make_float2(ax, ay);
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {
@@ -909,14 +921,23 @@ void launch_attraction(const Bufs& b, const Params& p, hipStream_t s) {
     int total = lb + mb + b.nattr_chunks;
     if (b.attr_grid > 0 && total > b.attr_grid) total = b.attr_grid;   // the launch grid
     if (total > 0) {
-        if (p.linlog || b.ew)
-            k_attract<true><<<total, 256, 0, s>>>(b.pos, b.mass, b.pos_rank, b.rowptr, b.col_rank, b.node_lists, b.nlight,
-                                                  b.nmedium, b.attr_chunks, b.nattr_chunks, lb, mb, p.strong_gravity,
-                                                  p.k_g, b.fga, b.attr_part, p.linlog, b.ew);
-        else
-            k_attract<false><<<total, 256, 0, s>>>(b.pos, b.mass, b.pos_rank, b.rowptr, b.col_rank, b.node_lists,
-                                                   b.nlight, b.nmedium, b.attr_chunks, b.nattr_chunks, lb, mb,
-                                                   p.strong_gravity, p.k_g, b.fga, b.attr_part, 0, nullptr);
+        // a grid smaller than the work (beside other kernels): every block stages the hubs'
+        // positions once and uses them for several virtual blocks
+        const bool hub = b.attr_grid > 0 && lb + mb + b.nattr_chunks > 2 * total;
+        const bool gen = p.linlog || b.ew;
+#define GFA2_ATTRACT(G, H)                                                                                   \
+    k_attract<G, H><<<total, 256, 0, s>>>(b.pos, b.mass, b.n, b.pos_rank, b.rowptr, b.col_rank, b.node_lists,   \
+                                          b.nlight, b.nmedium, b.attr_chunks, b.nattr_chunks, lb, mb,         \
+                                          p.strong_gravity, p.k_g, b.fga, b.attr_part, G ? p.linlog : 0,      \
+                                          G ? b.ew : nullptr)
+        if (gen) {
+            if (hub) GFA2_ATTRACT(true, true);
+            else GFA2_ATTRACT(true, false);
+        } else {
+            if (hub) GFA2_ATTRACT(false, true);
+            else GFA2_ATTRACT(false, false);
+        }
+#undef GFA2_ATTRACT
     }
     if (b.nheavy > 0)
         k_attract_heavy<<<(b.nheavy + 255) / 256, 256, 0, s>>>(b.pos, b.mass, b.node_lists + b.nlight + b.nmedium,
