@@ -75,10 +75,11 @@ def streaming_bytes(V, E):
 
 
 def walk_algorithmic_bytes(V, C, split):
-    """The walk's compulsory bytes per launch: every record (V bodies + C cells, 16 B) read once,
-    brec and perm 4 B per body, and F written (8 B) per body; the fused epilogue (one stream)
-    also reads F_ga and F' (8 B each), which the split walk leaves to the combine kernel."""
-    return 16 * (V + C) + 8 * V + (8 if split else 24) * V
+    """The walk's compulsory bytes per launch: every record (V bodies + C cells, 16 B) read once
+    and every cell's child list (16 B) once, each body's own {x, y, m} (16 B, the sorted copy)
+    and id (4 B), and F written (8 B) per body; the fused epilogue (one stream) also reads F_ga
+    and F' (8 B each), which the split walk leaves to the combine kernel."""
+    return 16 * (V + C) + 16 * C + 20 * V + (8 if split else 24) * V
 
 
 def cpu_baseline(g, pos, strong, k_g, k_r, iters, warm=1, keep=False):
